@@ -1,0 +1,86 @@
+"""Build libaqflip.so (HIP, gfx950) and the C++ host driver in-tree.
+
+    python -m aquarius_amd.build            # library + driver
+    python -m aquarius_amd.build --lib      # library only
+
+Outputs go to aquarius_amd/_build/. Every call compiles from source.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+OUT = PKG / "_build"
+INCLUDE = PKG.parent / "include"
+LIB = OUT / "libaqflip.so"
+DRIVER = OUT / "aqflip_driver"
+
+ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off keeps the reference kernels' rounding (no fused multiply-add),
+# and packed FP32 instructions are disabled for every HIP translation unit.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    f"-I{INCLUDE}",
+]
+
+LIB_SOURCES = ["flip_kernels.hip", "flip_api.cpp", "flip_seed.cpp"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[-1]}")
+
+
+def _compile(src: str) -> Path:
+    obj = OUT / (src.replace(".", "_") + ".o")
+    lang = [] if src.endswith(".hip") else ["-x", "hip"]
+    _run([HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)])
+    return obj
+
+
+def build_library() -> Path:
+    OUT.mkdir(parents=True, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        objs = list(ex.map(_compile, LIB_SOURCES))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_driver() -> Path:
+    """Headless C++ driver: the reference's init()/simulationStep() surface
+    (Fluid_3D_FLIP::Fluid) over the C ABI, linked against libaqflip.so."""
+    OUT.mkdir(parents=True, exist_ok=True)
+    tmp = DRIVER.with_suffix(".tmp")
+    _run(["g++", "-std=c++17", "-O2", f"-I{INCLUDE}", f"-I{CSRC / 'host'}",
+          str(CSRC / "host" / "aqflip_driver.cpp"), "-o", str(tmp),
+          f"-L{OUT}", "-laqflip", f"-Wl,-rpath,{OUT}", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
+def build(lib_only: bool = False) -> None:
+    build_library()
+    if not lib_only:
+        build_driver()
+
+
+if __name__ == "__main__":
+    build(lib_only="--lib" in sys.argv)
+    print(LIB)

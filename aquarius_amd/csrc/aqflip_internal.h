@@ -1,0 +1,100 @@
+// aqflip_internal.h -- device data layout and kernel launchers of libaqflip.
+//
+// HBM layout (one GPU, everything resident for the life of the handle):
+//   * Particles: structure of arrays, 10 float channels of Np each
+//     (px py pz vx vy vz f0 f1 f2 f3), two buffers (current and sorted copy).
+//     Only the first K fraction channels are read or written; K is the number of
+//     phases in use, the rest stay exactly zero.
+//   * Grid: every field uses one padded cell index over the face extent
+//     (X+1)x(Y+1)x(Z+1) of MAC_Grid_3D (Fluid/MAC_Grid_3D.cu:53-71):
+//     g(x,y,z) = (x*EY + y)*EZ + z with EY = Y+1 and EZ = Z+1 rounded up to 8, so
+//     z rows are 32-byte aligned and all fields share stencil offsets (sx, sy, 1).
+//     Cell-centred fields (pressure, divergence, density, fractions) use the
+//     interior x<X, y<Y, z<Z only. Face velocities are split per component
+//     (u, v, w) so each stencil reads one dense array.
+//   * Cell bins: counts, exclusive offsets (X*Y*Z + 1) over the particle hash
+//     x*Y*Z + y*Z + z (Fluid/Fluid_3D_kernels.cuh:8-27), and the permutation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace aqf {
+
+enum : uint8_t { AIR = 0, FLUID = 1, SOLID = 2 };  // Fluid/MAC_Grid_3D.cuh:15-17
+
+struct Geom {
+  int X, Y, Z;        // cells
+  int EY, EZ;         // padded strides: EY = Y+1, EZ = round_up(Z+1, 8)
+  long long sx;       // x stride = (long long)EY*EZ
+  long long ncell;    // (X+1)*EY*EZ
+  int ncells_hash;    // X*Y*Z
+  float h;            // cellPhysicalSize
+  __host__ __device__ long long g(int x, int y, int z) const {
+    return (long long)x * sx + (long long)y * EZ + z;
+  }
+};
+
+struct Particles {
+  float* c[10];  // px py pz vx vy vz f0 f1 f2 f3
+};
+
+struct Grid {
+  uint8_t* content;     // AIR/FLUID/SOLID
+  float* vel[3];        // velocity (P2G result, pre-gravity)
+  float* nvel[2][3];    // newVelocity, ping-pong for extrapolation
+  uint8_t* hasvel[2];   // hasVelocity bits 0..2, ping-pong
+  float* p[2];          // pressure, ping-pong
+  float* div;           // divergence
+  float* rhs;           // -divergence*density (Jacobi right-hand side)
+  float* density;
+  float* vf[4];         // volumeFractions
+  float* nvf[2][4];     // newVolumeFractions, ping-pong
+};
+
+struct Bins {
+  int* count;     // X*Y*Z per-cell counts (returns to zero after the scatter)
+  int* offsets;   // X*Y*Z + 1 exclusive prefix
+  int* partials;  // scan block partials
+  int* hash;      // per particle, previous order
+  int* perm;      // sorted slot -> previous index
+};
+
+struct StepParams {
+  float dt, flip;
+  float g[3];
+  float rho[4];
+  float lambda;  // D*dt/dx^2
+  int coverX, coverY, coverZ;  // fixBoundary pair coverage
+};
+
+// ---- launchers (flip_kernels.hip) ----
+void launch_fill_u32(void* p, uint32_t v, long long n, hipStream_t s);
+void launch_hash_count(const Geom& G, Particles P, long long Np, Bins B, hipStream_t s);
+void launch_scan(const Geom& G, Bins B, hipStream_t s);
+void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, long long Np, int K,
+                                Bins B, hipStream_t s);
+void launch_p2g(const Geom& G, Particles P, long long Np, int K, Bins B, Grid gr, int nv,
+                int hv, int nf, StepParams sp, hipStream_t s);
+void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int doGravity,
+                             int doBoundary, hipStream_t s);
+void launch_divergence(const Geom& G, Grid gr, int nv, Bins B, hipStream_t s);
+void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
+void launch_jacobi(const Geom& G, Grid gr, int pin, hipStream_t s);
+void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
+void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s);
+void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s);
+void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, hipStream_t s);
+void launch_g2p(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv, int nf,
+                StepParams sp, hipStream_t s);
+void launch_advect(const Geom& G, Particles P, long long Np, Grid gr, int nv, StepParams sp,
+                   hipStream_t s);
+void launch_g2p_advect_hash(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv,
+                            int nf, StepParams sp, Bins B, hipStream_t s);
+// FP64 reductions: residual terms (sum r^2, sum b^2) and volume statistics.
+void launch_residual(const Geom& G, Grid gr, int pb, double* partial, int nparts, double* out,
+                     hipStream_t s);
+void launch_stats(const Geom& G, Particles P, long long Np, Grid gr, double* partial, int nparts,
+                  double* out, hipStream_t s);
+
+}  // namespace aqf

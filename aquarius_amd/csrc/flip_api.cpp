@@ -1,0 +1,873 @@
+// flip_api.cpp -- C ABI of libaqflip (include/aqflip.h).
+//
+// Host orchestration of Fluid_3D_FLIP::Fluid (Fluid/Fluid_3D_FLIP.cu:316-505) for
+// one GPU: every stage is enqueued on the handle's own HIP stream with no host
+// synchronisation inside a step (the reference synchronises about 30 times per
+// step); entry points synchronise once before returning. Errors come back as
+// aq_status with a thread-local message instead of exit().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aqflip.h"
+#include "aqflip_internal.h"
+
+namespace aqf {
+int64_t seed_scene(const aq_flip_config& cfg, std::vector<float>* pos, std::vector<float>* frac);
+}
+
+using namespace aqf;
+
+namespace {
+
+thread_local std::string g_err;
+
+aq_status fail(aq_status s, const std::string& msg) {
+  g_err = msg;
+  return s;
+}
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(e_ == hipErrorOutOfMemory ? AQ_ENOMEM : AQ_EHIP,                            \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+  } while (0)
+
+#define AQ_TRY(expr)              \
+  do {                            \
+    aq_status s_ = (expr);        \
+    if (s_ != AQ_OK) return s_;   \
+  } while (0)
+
+int boundary_coverage(int A, int B, bool ref) {
+  const int pairs = (A + 1) * (B + 1);
+  if (!ref) return pairs;
+  const int total = A * B;
+  const int nt = std::min(1024, total);
+  // divUp (Common/Commons.h:7-13)
+  const int blocks = nt == 0 ? 1 : (total % nt != 0 ? total / nt + 1 : total / nt);
+  return (int)std::min<long long>((long long)blocks * nt, pairs);
+}
+
+}  // namespace
+
+struct aq_flip {
+  aq_flip_config cfg{};
+  Geom G{};
+  int K = 1;
+  long long Np = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+
+  Particles P[2]{};
+  int cur = 0;
+  Grid gr{};
+  int nv = 0, hv = 0, pb = 0, nf = 0;  // current buffer of each ping-pong pair
+  Bins B{};
+  StepParams sp{};
+  int nparts_red = 0;
+  double* red_partial = nullptr;
+  double* red_out = nullptr;
+  float physicalTime = 0.f;
+
+  bool timing = false;
+  double stage_ms[AQ_STAGE_COUNT]{};
+  int64_t stage_calls[AQ_STAGE_COUNT]{};
+  // Stage timing: event pairs recorded on this handle's stream while a step is
+  // enqueued, read back only at the synchronisation point that ends the call.
+  struct Mark {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> event_pool;
+  std::vector<Mark> pending;
+  size_t pool_used = 0;
+  hipEvent_t open_event = nullptr;
+
+  ~aq_flip() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void* p : allocs) hipFree(p);
+    for (hipEvent_t e : event_pool) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  aq_status alloc(void** p, size_t bytes) {
+    HIP_TRY(hipMalloc(p, bytes));
+    allocs.push_back(*p);
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes, stream));
+    return AQ_OK;
+  }
+  template <typename T>
+  aq_status allocT(T** p, size_t n) {
+    return alloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+  }
+
+  hipEvent_t next_event() {
+    if (pool_used == event_pool.size()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      event_pool.push_back(e);
+    }
+    return event_pool[pool_used++];
+  }
+  void begin(int st) {
+    (void)st;
+    if (!timing) return;
+    open_event = next_event();
+    if (open_event) hipEventRecord(open_event, stream);
+  }
+  void end(int st) {
+    if (!timing || !open_event) return;
+    hipEvent_t e = next_event();
+    if (!e) return;
+    hipEventRecord(e, stream);
+    pending.push_back({st, open_event, e});
+    open_event = nullptr;
+  }
+  // Called after the stream has been synchronised.
+  aq_status collect() {
+    for (const Mark& m : pending) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, m.a, m.b));
+      stage_ms[m.stage] += ms;
+      stage_calls[m.stage] += 1;
+    }
+    pending.clear();
+    pool_used = 0;
+    return AQ_OK;
+  }
+};
+
+namespace {
+
+aq_status validate(const aq_flip_config* c) {
+  if (!c) return fail(AQ_EINVAL, "null config");
+  for (int a = 0; a < 3; ++a)
+    if (c->size[a] < 1 || c->size[a] > 1024)
+      return fail(AQ_EINVAL, "grid size must be in [1, 1024] on every axis");
+  if ((long long)c->size[0] * c->size[1] * c->size[2] > (1ll << 30))
+    return fail(AQ_EINVAL, "grid too large");
+  if (c->phase_count > 4) return fail(AQ_EINVAL, "Only supports up to 4 phases");
+  if (c->phase_count < 1) return fail(AQ_EINVAL, "phase_count must be >= 1");
+  if (c->n_volumes < 0 || c->n_volumes > AQ_MAX_VOLUMES)
+    return fail(AQ_EINVAL, "n_volumes out of range");
+  if (!(c->grid_physical_size > 0) || !std::isfinite(c->grid_physical_size))
+    return fail(AQ_EINVAL, "grid_physical_size must be positive");
+  if (!std::isfinite(c->dt) || !std::isfinite(c->flip_coeff))
+    return fail(AQ_EINVAL, "dt and flip_coeff must be finite");
+  if (c->pressure_iters < 0 || c->diffusion_iters < 0)
+    return fail(AQ_EINVAL, "iteration counts must be >= 0");
+  if (c->pressure_init != AQ_PRESSURE_ZERO && c->pressure_init != AQ_PRESSURE_PREVIOUS)
+    return fail(AQ_EINVAL, "pressure_init must be AQ_PRESSURE_ZERO or AQ_PRESSURE_PREVIOUS");
+  return AQ_OK;
+}
+
+// Number of volume-fraction channels in use: phases 0..K-1 can be non-zero; the
+// remaining channels are provably zero throughout the step.
+int channels_for(const aq_flip_config& c, const float* frac4, long long n) {
+  int K = std::max(1, std::min(4, c.phase_count));
+  for (int i = 0; i < c.n_volumes; ++i)
+    if (c.volumes[i].phase >= 0 && c.volumes[i].phase < 4) K = std::max(K, c.volumes[i].phase + 1);
+  if (frac4)
+    for (long long i = 0; i < n; ++i)
+      for (int a = 3; a >= K; --a)
+        if (frac4[4 * i + a] != 0.f) {
+          K = a + 1;
+          break;
+        }
+  return K;
+}
+
+// Positions must hash into the grid (calcHashImpl would index out of range otherwise).
+aq_status check_positions(const aq_flip* h, const float* pos, long long n) {
+  const int S[3] = {h->G.X, h->G.Y, h->G.Z};
+  for (long long i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float v = pos[3 * i + a];
+      if (!(v >= 0.f) || (int)(v / h->G.h) >= S[a])
+        return fail(AQ_EINVAL, "particle " + std::to_string(i) + " lies outside the grid");
+    }
+  return AQ_OK;
+}
+
+aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
+  h->cfg = *cfg;
+  Geom& G = h->G;
+  G.X = cfg->size[0];
+  G.Y = cfg->size[1];
+  G.Z = cfg->size[2];
+  G.EY = G.Y + 1;
+  G.EZ = ((G.Z + 1) + 7) / 8 * 8;
+  G.sx = (long long)G.EY * G.EZ;
+  G.ncell = (long long)(G.X + 1) * G.sx;
+  G.ncells_hash = G.X * G.Y * G.Z;
+  G.h = cfg->grid_physical_size / (float)G.Y;  // Fluid_3D_FLIP.cu:444
+  h->Np = Np;
+
+  StepParams& sp = h->sp;
+  sp.dt = cfg->dt;
+  sp.flip = cfg->flip_coeff;
+  for (int a = 0; a < 3; ++a) sp.g[a] = cfg->gravity[a];
+  for (int a = 0; a < 4; ++a) sp.rho[a] = cfg->phase_densities[a];
+  sp.lambda = (cfg->diffusion_coeff * cfg->dt) / (G.h * G.h);  // Fluid_3D_common.cu:86
+  const bool ref = cfg->ref_boundary_coverage != 0;
+  sp.coverX = boundary_coverage(G.Y, G.Z, ref);
+  sp.coverY = boundary_coverage(G.X, G.Z, ref);
+  sp.coverZ = boundary_coverage(G.X, G.Y, ref);
+
+  HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (int b = 0; b < 2; ++b)
+    for (int ch = 0; ch < 10; ++ch) AQ_TRY(h->allocT(&h->P[b].c[ch], (size_t)Np));
+  const size_t nc = (size_t)G.ncell;
+  Grid& gr = h->gr;
+  AQ_TRY(h->allocT(&gr.content, nc));
+  for (int a = 0; a < 3; ++a) {
+    AQ_TRY(h->allocT(&gr.vel[a], nc));
+    AQ_TRY(h->allocT(&gr.nvel[0][a], nc));
+    AQ_TRY(h->allocT(&gr.nvel[1][a], nc));
+  }
+  AQ_TRY(h->allocT(&gr.hasvel[0], nc));
+  AQ_TRY(h->allocT(&gr.hasvel[1], nc));
+  AQ_TRY(h->allocT(&gr.p[0], nc));
+  AQ_TRY(h->allocT(&gr.p[1], nc));
+  AQ_TRY(h->allocT(&gr.div, nc));
+  AQ_TRY(h->allocT(&gr.rhs, nc));
+  AQ_TRY(h->allocT(&gr.density, nc));
+  for (int a = 0; a < 4; ++a) {
+    AQ_TRY(h->allocT(&gr.vf[a], nc));
+    AQ_TRY(h->allocT(&gr.nvf[0][a], nc));
+    AQ_TRY(h->allocT(&gr.nvf[1][a], nc));
+  }
+  const int nch = G.ncells_hash;
+  AQ_TRY(h->allocT(&h->B.count, (size_t)nch));
+  AQ_TRY(h->allocT(&h->B.offsets, (size_t)nch + 1));
+  AQ_TRY(h->allocT(&h->B.partials, (size_t)(nch + 4095) / 4096 + 1));
+  AQ_TRY(h->allocT(&h->B.hash, (size_t)Np));
+  AQ_TRY(h->allocT(&h->B.perm, (size_t)Np));
+  h->nparts_red = 512;
+  AQ_TRY(h->allocT(&h->red_partial, (size_t)h->nparts_red * 16));
+  AQ_TRY(h->allocT(&h->red_out, 16));
+  return AQ_OK;
+}
+
+aq_status upload_particles(aq_flip* h, const float* pos, const float* vel, const float* frac4) {
+  const long long n = h->Np;
+  std::vector<float> chan((size_t)n);
+  Particles& P = h->P[h->cur];
+  for (int a = 0; a < 3; ++a) {
+    for (long long i = 0; i < n; ++i) chan[i] = pos[3 * i + a];
+    HIP_TRY(hipMemcpyAsync(P.c[a], chan.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (long long i = 0; i < n; ++i) chan[i] = vel ? vel[3 * i + a] : 0.f;
+    HIP_TRY(hipMemcpyAsync(P.c[3 + a], chan.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  for (int a = 0; a < 4; ++a) {
+    for (long long i = 0; i < n; ++i) chan[i] = frac4 ? frac4[4 * i + a] : 0.f;
+    HIP_TRY(hipMemcpyAsync(P.c[6 + a], chan.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  return AQ_OK;
+}
+
+// ---- stages --------------------------------------------------------------
+
+// hash is already binned (count filled): scan, scatter, segment sort, gather, P2G.
+void transfer_after_count(aq_flip* h) {
+  launch_scan(h->G, h->B, h->stream);
+  launch_scatter_sort_gather(h->G, h->P[h->cur], h->P[h->cur ^ 1], h->Np, h->K, h->B, h->stream);
+  h->cur ^= 1;
+  launch_p2g(h->G, h->P[h->cur], h->Np, h->K, h->B, h->gr, h->nv, h->hv, h->nf, h->sp, h->stream);
+}
+
+void clear_bins(aq_flip* h) { launch_fill_u32(h->B.count, 0u, h->G.ncells_hash, h->stream); }
+
+void stage_transfer(aq_flip* h) {
+  clear_bins(h);
+  launch_hash_count(h->G, h->P[h->cur], h->Np, h->B, h->stream);
+  transfer_after_count(h);
+}
+
+void jacobi_sweeps(aq_flip* h, int n) {
+  for (int i = 0; i < n; ++i) {
+    launch_jacobi(h->G, h->gr, h->pb, h->stream);
+    h->pb ^= 1;
+  }
+}
+
+void stage_pressure(aq_flip* h) {
+  if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
+  jacobi_sweeps(h, h->cfg.pressure_iters);
+}
+
+void stage_extrapolate(aq_flip* h) {
+  for (int i = 0; i < 3; ++i) {  // maxDistance = 3 (Fluid_3D_common.cu:74)
+    launch_extrapolate(h->G, h->gr, h->nv, h->hv, h->stream);
+    h->nv ^= 1;
+    h->hv ^= 1;
+  }
+}
+
+// inStep: the grid fractions were produced by this step's P2G, so with a single
+// channel they are exactly 1 (FLUID) or 0 (AIR) and every sweep is the identity;
+// Jacobi with >= 1 sweep has already zeroed AIR pressure.
+void stage_diffusion(aq_flip* h, bool inStep) {
+  const int iters = h->cfg.diffusion_iters;
+  if (iters <= 0) return;
+  if (!inStep || h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
+  if (inStep && h->K == 1) return;
+  for (int i = 0; i < iters; ++i) {
+    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->stream);
+    h->nf ^= 1;
+  }
+}
+
+aq_status run_stage(aq_flip* h, int st) {
+  h->begin(st);
+  switch (st) {
+    case AQ_STAGE_G2P: launch_g2p(h->G, h->P[h->cur], h->Np, h->K, h->gr, h->nv, h->nf, h->sp, h->stream); break;
+    case AQ_STAGE_ADVECT: launch_advect(h->G, h->P[h->cur], h->Np, h->gr, h->nv, h->sp, h->stream); break;
+    case AQ_STAGE_TRANSFER_TO_GRID: stage_transfer(h); break;
+    case AQ_STAGE_GRAVITY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 1, 0, h->stream); break;
+    case AQ_STAGE_BOUNDARY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 0, 1, h->stream); break;
+    case AQ_STAGE_DIVERGENCE: launch_divergence(h->G, h->gr, h->nv, h->B, h->stream); break;
+    case AQ_STAGE_PRESSURE: stage_pressure(h); break;
+    case AQ_STAGE_PROJECT: launch_project(h->G, h->gr, h->nv, h->hv, h->pb, h->stream); break;
+    case AQ_STAGE_EXTRAPOLATE: stage_extrapolate(h); break;
+    case AQ_STAGE_DIFFUSION: stage_diffusion(h, false); break;
+    default: return fail(AQ_EINVAL, "unknown stage");
+  }
+  HIP_TRY(hipGetLastError());
+  h->end(st);
+  return AQ_OK;
+}
+
+// Fluid::simulationStep (Fluid/Fluid_3D_FLIP.cu:316-352). G2P, advection, hashing
+// and bin counting are one particle pass; gravity and boundary one face pass.
+aq_status enqueue_step(aq_flip* h) {
+  clear_bins(h);
+  h->begin(AQ_STAGE_G2P);
+  launch_g2p_advect_hash(h->G, h->P[h->cur], h->Np, h->K, h->gr, h->nv, h->nf, h->sp, h->B, h->stream);
+  h->end(AQ_STAGE_G2P);
+  h->begin(AQ_STAGE_TRANSFER_TO_GRID);
+  transfer_after_count(h);
+  h->end(AQ_STAGE_TRANSFER_TO_GRID);
+  h->begin(AQ_STAGE_GRAVITY);
+  launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 1, 1, h->stream);
+  h->end(AQ_STAGE_GRAVITY);
+  h->begin(AQ_STAGE_DIVERGENCE);
+  launch_divergence(h->G, h->gr, h->nv, h->B, h->stream);
+  h->end(AQ_STAGE_DIVERGENCE);
+  if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
+  h->begin(AQ_STAGE_PRESSURE);  // the Jacobi sweeps only
+  jacobi_sweeps(h, h->cfg.pressure_iters);
+  h->end(AQ_STAGE_PRESSURE);
+  h->begin(AQ_STAGE_PROJECT);
+  launch_project(h->G, h->gr, h->nv, h->hv, h->pb, h->stream);
+  h->end(AQ_STAGE_PROJECT);
+  h->begin(AQ_STAGE_EXTRAPOLATE);
+  stage_extrapolate(h);
+  h->end(AQ_STAGE_EXTRAPOLATE);
+  h->begin(AQ_STAGE_DIFFUSION);
+  stage_diffusion(h, true);
+  h->end(AQ_STAGE_DIFFUSION);
+  HIP_TRY(hipGetLastError());
+  h->physicalTime += h->cfg.dt;
+  return AQ_OK;
+}
+
+aq_status create_common(const aq_flip_config* cfg, const float* pos, const float* vel,
+                        const float* frac4, long long n, aq_flip** out) {
+  aq_flip* h = new aq_flip;
+  aq_status st = setup(h, cfg, n);
+  if (st == AQ_OK) st = check_positions(h, pos, n);
+  if (st == AQ_OK) {
+    h->K = channels_for(*cfg, frac4, n);
+    st = upload_particles(h, pos, vel, frac4);
+  }
+  if (st == AQ_OK) {
+    // Fluid::init: fixBoundary, then transferToGrid (Fluid_3D_FLIP.cu:485,500).
+    launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 0, 1, h->stream);
+    stage_transfer(h);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) st = fail(AQ_EHIP, std::string("init: ") + hipGetErrorString(e));
+  }
+  if (st != AQ_OK) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return AQ_OK;
+}
+
+size_t field_elems(const aq_flip* h, int field, int* width, bool* isInt) {
+  const Geom& G = h->G;
+  const size_t e1 = (size_t)(G.X + 1) * (G.Y + 1) * (G.Z + 1), e0 = (size_t)G.X * G.Y * G.Z;
+  *isInt = false;
+  switch (field) {
+    case AQ_FIELD_CONTENT: *width = 1; *isInt = true; return e1;
+    case AQ_FIELD_VELOCITY:
+    case AQ_FIELD_NEW_VELOCITY: *width = 4; return e1;
+    case AQ_FIELD_PRESSURE:
+    case AQ_FIELD_DIVERGENCE:
+    case AQ_FIELD_DENSITY: *width = 1; return e0;
+    case AQ_FIELD_PARTICLE_COUNT: *width = 1; *isInt = true; return e0;
+    case AQ_FIELD_HAS_VELOCITY: *width = 4; *isInt = true; return e0;
+    case AQ_FIELD_VOLUME_FRACTIONS:
+    case AQ_FIELD_NEW_VOLUME_FRACTIONS: *width = 4; return e0;
+  }
+  *width = 0;
+  return 0;
+}
+
+template <typename T>
+aq_status d2h(aq_flip* h, std::vector<T>& dst, const void* src, size_t n) {
+  dst.resize(n);
+  HIP_TRY(hipMemcpyAsync(dst.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return AQ_OK;
+}
+template <typename T>
+aq_status h2d(aq_flip* h, void* dst, const std::vector<T>& src) {
+  HIP_TRY(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return AQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+aq_status aq_flip_config_default(aq_flip_config* c) {
+  if (!c) return fail(AQ_EINVAL, "null config");
+  std::memset(c, 0, sizeof(*c));
+  // ConfigFLIP (Fluid/FluidConfig.cuh:41-50)
+  c->size[0] = c->size[1] = c->size[2] = 50;
+  c->pressure_iters = 100;
+  c->diffusion_iters = 100;
+  c->dt = 0.033f;
+  c->flip_coeff = 0.95f;
+  c->particle_jitter = 0.05f;
+  c->grid_physical_size = 10.f;  // Fluid_3D_FLIP.cuh:62
+  // FluidConfig (Fluid/FluidConfig.cuh:68-97)
+  c->gravity[0] = 0.f;
+  c->gravity[1] = -9.8f;
+  c->gravity[2] = 0.f;
+  c->phase_count = 2;
+  for (int a = 0; a < 4; ++a) c->phase_densities[a] = 1.f;
+  c->diffusion_coeff = 0.001f;
+  c->pressure_init = AQ_PRESSURE_PREVIOUS;
+  c->ref_boundary_coverage = 1;
+  c->seed = 0;
+  // Default scene: a pool and a ball (InitializationVolume, FluidConfig.cuh:20-39).
+  c->n_volumes = 2;
+  aq_flip_volume& box = c->volumes[0];
+  box.shape = AQ_SHAPE_BOX;
+  box.box_min[0] = box.box_min[1] = box.box_min[2] = 0.f;
+  box.box_max[0] = 1.f;
+  box.box_max[1] = 0.1875f;
+  box.box_max[2] = 1.f;
+  box.center[0] = 0.5f;
+  box.center[1] = 0.75f;
+  box.center[2] = 0.5f;
+  box.radius = 0.125f;
+  box.phase = 0;
+  c->volumes[1] = box;
+  c->volumes[1].shape = AQ_SHAPE_SPHERE;
+  return AQ_OK;
+}
+
+aq_status aq_flip_seed_particles(const aq_flip_config* cfg, float* pos, float* frac4,
+                                 int64_t capacity, int64_t* n_out) {
+  AQ_TRY(validate(cfg));
+  if (!n_out) return fail(AQ_EINVAL, "null n_out");
+  if (!pos) {
+    *n_out = seed_scene(*cfg, nullptr, nullptr);
+    return AQ_OK;
+  }
+  std::vector<float> p, f;
+  const int64_t n = seed_scene(*cfg, &p, &f);
+  *n_out = n;
+  if (capacity < n) return fail(AQ_EINVAL, "capacity smaller than the particle count");
+  std::memcpy(pos, p.data(), p.size() * sizeof(float));
+  if (frac4) std::memcpy(frac4, f.data(), f.size() * sizeof(float));
+  return AQ_OK;
+}
+
+aq_status aq_flip_create(const aq_flip_config* cfg, aq_flip** out) {
+  AQ_TRY(validate(cfg));
+  if (!out) return fail(AQ_EINVAL, "null out");
+  std::vector<float> p, f;
+  const int64_t n = seed_scene(*cfg, &p, &f);
+  if (n <= 0) return fail(AQ_EINVAL, "the configured volumes produce no particles");
+  return create_common(cfg, p.data(), nullptr, f.data(), n, out);
+}
+
+aq_status aq_flip_create_with_particles(const aq_flip_config* cfg, const float* pos,
+                                        const float* vel, const float* frac4, int64_t n,
+                                        aq_flip** out) {
+  AQ_TRY(validate(cfg));
+  if (!out || !pos) return fail(AQ_EINVAL, "null argument");
+  if (n <= 0 || n > (1ll << 31) - 1) return fail(AQ_EINVAL, "particle count out of range");
+  return create_common(cfg, pos, vel, frac4, n, out);
+}
+
+void aq_flip_destroy(aq_flip* h) { delete h; }
+
+aq_status aq_flip_step(aq_flip* h) {
+  if (!h) return fail(AQ_EINVAL, "null handle");
+  AQ_TRY(enqueue_step(h));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return h->collect();
+}
+
+aq_status aq_flip_steps(aq_flip* h, int32_t count) {
+  if (!h || count < 0) return fail(AQ_EINVAL, "bad argument");
+  for (int i = 0; i < count; ++i) AQ_TRY(enqueue_step(h));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return h->collect();
+}
+
+aq_status aq_flip_run_stage(aq_flip* h, int32_t stage) {
+  if (!h) return fail(AQ_EINVAL, "null handle");
+  if (stage < 0 || stage >= AQ_STAGE_COUNT) return fail(AQ_EINVAL, "unknown stage");
+  AQ_TRY(run_stage(h, stage));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return h->collect();
+}
+
+int64_t aq_flip_particle_count(const aq_flip* h) { return h ? h->Np : 0; }
+
+aq_status aq_flip_get_particles(aq_flip* h, float* pos, float* vel, float* frac4) {
+  if (!h) return fail(AQ_EINVAL, "null handle");
+  const long long n = h->Np;
+  std::vector<float> ch;
+  const Particles& P = h->P[h->cur];
+  for (int c = 0; c < 10; ++c) {
+    float* dst = c < 3 ? pos : (c < 6 ? vel : frac4);
+    if (!dst) continue;
+    const int w = c < 6 ? 3 : 4, a = c < 3 ? c : (c < 6 ? c - 3 : c - 6);
+    if (c >= 6 && a >= h->K) {
+      for (long long i = 0; i < n; ++i) dst[w * i + a] = 0.f;
+      continue;
+    }
+    AQ_TRY(d2h(h, ch, P.c[c], (size_t)n));
+    for (long long i = 0; i < n; ++i) dst[w * i + a] = ch[i];
+  }
+  return AQ_OK;
+}
+
+aq_status aq_flip_set_particles(aq_flip* h, const float* pos, const float* vel, const float* frac4,
+                                int64_t n) {
+  if (!h || !pos) return fail(AQ_EINVAL, "null argument");
+  if (n != h->Np) return fail(AQ_EINVAL, "particle count differs from the handle's");
+  AQ_TRY(check_positions(h, pos, n));
+  h->K = std::max(h->K, channels_for(h->cfg, frac4, n));
+  return upload_particles(h, pos, vel, frac4);
+}
+
+size_t aq_flip_field_bytes(const aq_flip* h, int32_t field) {
+  if (!h) return 0;
+  int w;
+  bool isInt;
+  const size_t n = field_elems(h, field, &w, &isInt);
+  return n * (size_t)w * 4;
+}
+
+aq_status aq_flip_get_field(aq_flip* h, int32_t field, void* dst, size_t bytes) {
+  if (!h || !dst) return fail(AQ_EINVAL, "null argument");
+  int w;
+  bool isInt;
+  const size_t n = field_elems(h, field, &w, &isInt);
+  if (n == 0) return fail(AQ_EINVAL, "unknown field");
+  if (bytes != n * w * 4) return fail(AQ_EINVAL, "byte count does not match the field");
+  const Geom& G = h->G;
+  const size_t nc = (size_t)G.ncell;
+  float* F = (float*)dst;
+  int32_t* I = (int32_t*)dst;
+  const bool face = (field == AQ_FIELD_CONTENT || field == AQ_FIELD_VELOCITY ||
+                     field == AQ_FIELD_NEW_VELOCITY);
+  const int EX = face ? G.X + 1 : G.X, EYe = face ? G.Y + 1 : G.Y, EZe = face ? G.Z + 1 : G.Z;
+  auto each = [&](auto&& fn) {
+    size_t o = 0;
+    for (int x = 0; x < EX; ++x)
+      for (int y = 0; y < EYe; ++y)
+        for (int z = 0; z < EZe; ++z, ++o) fn(o, (size_t)G.g(x, y, z));
+  };
+  switch (field) {
+    case AQ_FIELD_CONTENT: {
+      std::vector<uint8_t> c;
+      AQ_TRY(d2h(h, c, h->gr.content, nc));
+      each([&](size_t o, size_t g) { I[o] = c[g]; });
+      break;
+    }
+    case AQ_FIELD_VELOCITY:
+    case AQ_FIELD_NEW_VELOCITY: {
+      std::vector<float> comp[3];
+      for (int a = 0; a < 3; ++a)
+        AQ_TRY(d2h(h, comp[a], field == AQ_FIELD_VELOCITY ? h->gr.vel[a] : h->gr.nvel[h->nv][a], nc));
+      each([&](size_t o, size_t g) {
+        for (int a = 0; a < 3; ++a) F[4 * o + a] = comp[a][g];
+        F[4 * o + 3] = 0.f;
+      });
+      break;
+    }
+    case AQ_FIELD_PRESSURE:
+    case AQ_FIELD_DIVERGENCE:
+    case AQ_FIELD_DENSITY: {
+      const float* src = field == AQ_FIELD_PRESSURE ? h->gr.p[h->pb]
+                         : field == AQ_FIELD_DIVERGENCE ? h->gr.div
+                                                        : h->gr.density;
+      std::vector<float> v;
+      AQ_TRY(d2h(h, v, src, nc));
+      each([&](size_t o, size_t g) { F[o] = v[g]; });
+      break;
+    }
+    case AQ_FIELD_PARTICLE_COUNT: {
+      std::vector<int> off;
+      AQ_TRY(d2h(h, off, h->B.offsets, (size_t)G.ncells_hash + 1));
+      for (int c = 0; c < G.ncells_hash; ++c) {
+        const int cnt = off[c + 1] - off[c];
+        I[c] = cnt > 0 ? cnt - 1 : 0;  // cellEnd - cellStart (Fluid_3D_FLIP.cu:180)
+      }
+      break;
+    }
+    case AQ_FIELD_HAS_VELOCITY: {
+      std::vector<uint8_t> v;
+      AQ_TRY(d2h(h, v, h->gr.hasvel[h->hv], nc));
+      each([&](size_t o, size_t g) {
+        for (int a = 0; a < 3; ++a) I[4 * o + a] = (v[g] >> a) & 1;
+        I[4 * o + 3] = 0;
+      });
+      break;
+    }
+    case AQ_FIELD_VOLUME_FRACTIONS:
+    case AQ_FIELD_NEW_VOLUME_FRACTIONS: {
+      std::vector<float> comp[4];
+      for (int a = 0; a < h->K; ++a)
+        AQ_TRY(d2h(h, comp[a], field == AQ_FIELD_VOLUME_FRACTIONS ? h->gr.vf[a] : h->gr.nvf[h->nf][a], nc));
+      each([&](size_t o, size_t g) {
+        for (int a = 0; a < 4; ++a) F[4 * o + a] = a < h->K ? comp[a][g] : 0.f;
+      });
+      break;
+    }
+  }
+  return AQ_OK;
+}
+
+aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t bytes) {
+  if (!h || !src) return fail(AQ_EINVAL, "null argument");
+  int w;
+  bool isInt;
+  const size_t n = field_elems(h, field, &w, &isInt);
+  if (n == 0) return fail(AQ_EINVAL, "unknown field");
+  if (field == AQ_FIELD_PARTICLE_COUNT)
+    return fail(AQ_EINVAL, "particleCount is derived from the cell bins and cannot be set");
+  if (bytes != n * w * 4) return fail(AQ_EINVAL, "byte count does not match the field");
+  const Geom& G = h->G;
+  const size_t nc = (size_t)G.ncell;
+  const float* F = (const float*)src;
+  const int32_t* I = (const int32_t*)src;
+  const bool face = (field == AQ_FIELD_CONTENT || field == AQ_FIELD_VELOCITY ||
+                     field == AQ_FIELD_NEW_VELOCITY);
+  const int EX = face ? G.X + 1 : G.X, EYe = face ? G.Y + 1 : G.Y, EZe = face ? G.Z + 1 : G.Z;
+  auto each = [&](auto&& fn) {
+    size_t o = 0;
+    for (int x = 0; x < EX; ++x)
+      for (int y = 0; y < EYe; ++y)
+        for (int z = 0; z < EZe; ++z, ++o) fn(o, (size_t)G.g(x, y, z));
+  };
+  switch (field) {
+    case AQ_FIELD_CONTENT: {
+      std::vector<uint8_t> c;
+      AQ_TRY(d2h(h, c, h->gr.content, nc));
+      each([&](size_t o, size_t g) { c[g] = (uint8_t)I[o]; });
+      AQ_TRY(h2d(h, h->gr.content, c));
+      break;
+    }
+    case AQ_FIELD_VELOCITY:
+    case AQ_FIELD_NEW_VELOCITY: {
+      for (int a = 0; a < 3; ++a) {
+        float* d = field == AQ_FIELD_VELOCITY ? h->gr.vel[a] : h->gr.nvel[h->nv][a];
+        std::vector<float> v;
+        AQ_TRY(d2h(h, v, d, nc));
+        each([&](size_t o, size_t g) { v[g] = F[4 * o + a]; });
+        AQ_TRY(h2d(h, d, v));
+      }
+      break;
+    }
+    case AQ_FIELD_PRESSURE:
+    case AQ_FIELD_DIVERGENCE:
+    case AQ_FIELD_DENSITY: {
+      float* d = field == AQ_FIELD_PRESSURE ? h->gr.p[h->pb]
+                 : field == AQ_FIELD_DIVERGENCE ? h->gr.div
+                                                : h->gr.density;
+      std::vector<float> v;
+      AQ_TRY(d2h(h, v, d, nc));
+      each([&](size_t o, size_t g) { v[g] = F[o]; });
+      AQ_TRY(h2d(h, d, v));
+      if (field != AQ_FIELD_PRESSURE) {
+        // keep the Jacobi right-hand side consistent with divergence*density
+        std::vector<float> dv, de;
+        AQ_TRY(d2h(h, dv, h->gr.div, nc));
+        AQ_TRY(d2h(h, de, h->gr.density, nc));
+        for (size_t i = 0; i < nc; ++i) dv[i] = -dv[i] * de[i];
+        AQ_TRY(h2d(h, h->gr.rhs, dv));
+      }
+      break;
+    }
+    case AQ_FIELD_HAS_VELOCITY: {
+      std::vector<uint8_t> v;
+      AQ_TRY(d2h(h, v, h->gr.hasvel[h->hv], nc));
+      each([&](size_t o, size_t g) {
+        v[g] = (uint8_t)((I[4 * o] ? 1 : 0) | (I[4 * o + 1] ? 2 : 0) | (I[4 * o + 2] ? 4 : 0));
+      });
+      AQ_TRY(h2d(h, h->gr.hasvel[h->hv], v));
+      break;
+    }
+    case AQ_FIELD_VOLUME_FRACTIONS:
+    case AQ_FIELD_NEW_VOLUME_FRACTIONS: {
+      int need = 1;
+      for (size_t o = 0; o < n; ++o)
+        for (int a = 3; a >= need; --a)
+          if (F[4 * o + a] != 0.f) {
+            need = a + 1;
+            break;
+          }
+      h->K = std::max(h->K, need);
+      for (int a = 0; a < h->K; ++a) {
+        float* d = field == AQ_FIELD_VOLUME_FRACTIONS ? h->gr.vf[a] : h->gr.nvf[h->nf][a];
+        std::vector<float> v;
+        AQ_TRY(d2h(h, v, d, nc));
+        each([&](size_t o, size_t g) { v[g] = F[4 * o + a]; });
+        AQ_TRY(h2d(h, d, v));
+      }
+      break;
+    }
+  }
+  return AQ_OK;
+}
+
+aq_status aq_flip_get_cells(aq_flip* h, int32_t* cell_start, int32_t* cell_end, int32_t* hash,
+                            int32_t* perm) {
+  if (!h) return fail(AQ_EINVAL, "null handle");
+  const Geom& G = h->G;
+  std::vector<int> off;
+  AQ_TRY(d2h(h, off, h->B.offsets, (size_t)G.ncells_hash + 1));
+  const size_t e1 = (size_t)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
+  if (cell_start || cell_end) {
+    // Reference convention: cellStart/cellEnd sized (X+1)(Y+1)(Z+1), inclusive end,
+    // -1 where empty (Fluid_3D_common.cuh:47-48, Fluid_kernels.cuh:3-22).
+    for (size_t c = 0; c < e1; ++c) {
+      int s = -1, e = -1;
+      if (c < (size_t)G.ncells_hash && off[c + 1] > off[c]) {
+        s = off[c];
+        e = off[c + 1] - 1;
+      }
+      if (cell_start) cell_start[c] = s;
+      if (cell_end) cell_end[c] = e;
+    }
+  }
+  if (hash)
+    for (int c = 0; c < G.ncells_hash; ++c)
+      for (int k = off[c]; k < off[c + 1]; ++k) hash[k] = c;
+  if (perm) {
+    std::vector<int> p;
+    AQ_TRY(d2h(h, p, h->B.perm, (size_t)h->Np));
+    std::memcpy(perm, p.data(), p.size() * sizeof(int));
+  }
+  return AQ_OK;
+}
+
+aq_status aq_flip_residual_fp64(aq_flip* h, double* rel) {
+  if (!h || !rel) return fail(AQ_EINVAL, "null argument");
+  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  HIP_TRY(hipGetLastError());
+  double o[2];
+  HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  *rel = o[1] > 0 ? std::sqrt(o[0]) / std::sqrt(o[1]) : std::sqrt(o[0]);
+  return AQ_OK;
+}
+
+aq_status aq_flip_volume_stats(aq_flip* h, aq_flip_stats* out) {
+  if (!h || !out) return fail(AQ_EINVAL, "null argument");
+  launch_stats(h->G, h->P[h->cur], h->Np, h->gr, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  HIP_TRY(hipGetLastError());
+  double o[16];
+  HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  out->particles = h->Np;
+  out->fluid_cells = (int64_t)o[4];
+  for (int a = 0; a < 4; ++a) out->phase_sum[a] = a < h->K ? o[a] : 0.0;
+  out->mean_ppc = out->fluid_cells > 0 ? (double)h->Np / (double)out->fluid_cells : 0.0;
+  for (int a = 0; a < 3; ++a) {
+    out->pos_min[a] = -o[5 + a];
+    out->pos_max[a] = o[8 + a];
+  }
+  return AQ_OK;
+}
+
+aq_status aq_flip_solve_pressure(aq_flip* h, double tol, int32_t max_iters, int32_t check_every,
+                                 int32_t* iters_out, double* rel_out) {
+  if (!h || max_iters < 0 || check_every < 1) return fail(AQ_EINVAL, "bad argument");
+  int it = 0;
+  double rel = 0;
+  AQ_TRY(aq_flip_residual_fp64(h, &rel));
+  while (!(rel <= tol) && it < max_iters) {
+    const int k = std::min(check_every, max_iters - it);
+    jacobi_sweeps(h, k);
+    it += k;
+    AQ_TRY(aq_flip_residual_fp64(h, &rel));
+  }
+  if (iters_out) *iters_out = it;
+  if (rel_out) *rel_out = rel;
+  if (!(rel <= tol)) return fail(AQ_ESTALL, "pressure solve reached its iteration cap");
+  return AQ_OK;
+}
+
+aq_status aq_flip_set_timing(aq_flip* h, int32_t enable) {
+  if (!h) return fail(AQ_EINVAL, "null handle");
+  h->timing = enable != 0;
+  return AQ_OK;
+}
+aq_status aq_flip_get_timing(aq_flip* h, double* stage_ms, int64_t* stage_calls) {
+  if (!h) return fail(AQ_EINVAL, "null handle");
+  for (int i = 0; i < AQ_STAGE_COUNT; ++i) {
+    if (stage_ms) stage_ms[i] = h->stage_ms[i];
+    if (stage_calls) stage_calls[i] = h->stage_calls[i];
+  }
+  return AQ_OK;
+}
+aq_status aq_flip_reset_timing(aq_flip* h) {
+  if (!h) return fail(AQ_EINVAL, "null handle");
+  for (int i = 0; i < AQ_STAGE_COUNT; ++i) {
+    h->stage_ms[i] = 0;
+    h->stage_calls[i] = 0;
+  }
+  return AQ_OK;
+}
+
+double aq_flip_physical_time(const aq_flip* h) { return h ? (double)h->physicalTime : 0.0; }
+
+aq_status aq_flip_get_center(const aq_flip* h, float* center) {
+  if (!h || !center) return fail(AQ_EINVAL, "null argument");
+  // Fluid::getCenter returns gridPhysicalSize/2 on every axis (Fluid_3D_FLIP.cu:728-730).
+  for (int a = 0; a < 3; ++a) center[a] = h->cfg.grid_physical_size / 2;
+  return AQ_OK;
+}
+float aq_flip_container_size(const aq_flip* h) { return h ? h->cfg.grid_physical_size : 0.f; }
+
+const char* aq_last_error(void) { return g_err.c_str(); }
+
+const char* aq_flip_version(void) { return "aqflip 0.1.0 (gfx950)"; }
+
+}  // extern "C"

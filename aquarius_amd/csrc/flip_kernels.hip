@@ -1,0 +1,975 @@
+// flip_kernels.hip -- CDNA4 (gfx950) kernels of the FLIP simulation step.
+//
+// Every kernel keeps the floating-point operation order of the reference kernel
+// it replaces (cited per kernel), compiled without FMA contraction, so results
+// match the CPU oracle bit for bit where the arithmetic allows. Iterative solvers
+// read the previous iterate and write a new buffer (ping-pong) instead of the
+// reference's racy in-place texture/surface update.
+#include <hip/hip_runtime.h>
+
+#include "aqflip_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace aqf {
+
+constexpr int kBlock = 256;
+constexpr int kXCD = 8;
+
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs. Map the
+// hardware block id so that each XCD walks one contiguous range of logical
+// blocks: a slab of cells (or a run of sorted particles) then stays in one XCD's
+// L2 for the whole launch and across consecutive launches of the same shape.
+__device__ __forceinline__ long long xcd_block() {
+  const int nb = gridDim.x, bid = blockIdx.x;
+  const int per = nb / kXCD, rem = nb % kXCD;
+  const int xcd = bid % kXCD, k = bid / kXCD;
+  return (long long)xcd * per + min(xcd, rem) + k;
+}
+__device__ __forceinline__ long long gtid() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+static inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------------ helpers
+// trilinearUnitKernel / trilinearHatKernel(float3) (Fluid/WeightKernels.cuh:131-146).
+__device__ __forceinline__ float hat_unit(float r) {
+  r = fabsf(r);
+  if (r > 1.f) return 0.f;
+  return 1.f - r;
+}
+__device__ __forceinline__ float hat3(float rx, float ry, float rz, float s) {
+  return hat_unit(rx / s) * hat_unit(ry / s) * hat_unit(rz / s);
+}
+__device__ __forceinline__ float lerpf_(float a, float b, float t) { return a + t * (b - a); }
+
+// Interior cell (x,y,z) of a flat z-fastest index over X*Y*Z.
+__device__ __forceinline__ void cell_of(const Geom& G, int n, int& x, int& y, int& z) {
+  const int yz = G.Y * G.Z;
+  x = n / yz;
+  const int r = n - x * yz;
+  y = r / G.Z;
+  z = r - y * G.Z;
+}
+// Face cell (x,y,z) of a flat z-fastest index over (X+1)*(Y+1)*(Z+1).
+__device__ __forceinline__ void face_of(const Geom& G, int n, int& x, int& y, int& z) {
+  const int Z1 = G.Z + 1, yz = (G.Y + 1) * Z1;
+  x = n / yz;
+  const int r = n - x * yz;
+  y = r / Z1;
+  z = r - y * Z1;
+}
+
+// ------------------------------------------------------------------ fill
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+void launch_fill_u32(void* p, uint32_t v, long long n, hipStream_t s) {
+  int nb = (int)std::min<long long>((n + kBlock - 1) / kBlock, 8192);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_fill_u32, dim3(nb), dim3(kBlock), 0, s, (uint32_t*)p, v, n);
+}
+
+// ------------------------------------------------------------------ hashing
+// calcHashImpl (Fluid/Fluid_3D_kernels.cuh:8-27): truncating float->int of pos/dx.
+// Positions are always inside [dx/4, L - dx/4] after advection and validated on
+// upload, so the clamp to the grid is a no-op there; it only keeps a non-finite
+// position (from a non-finite velocity) from indexing outside the bins.
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+__device__ __forceinline__ int cell_hash(const Geom& G, float px, float py, float pz) {
+  const int x = clampi((int)(px / G.h), 0, G.X - 1);
+  const int y = clampi((int)(py / G.h), 0, G.Y - 1);
+  const int z = clampi((int)(pz / G.h), 0, G.Z - 1);
+  return x * (G.Y * G.Z) + y * G.Z + z;
+}
+
+__global__ void k_hash_count(Geom G, const float* __restrict__ px, const float* __restrict__ py,
+                             const float* __restrict__ pz, long long Np, int* __restrict__ hash,
+                             int* __restrict__ count) {
+  const long long i = gtid();
+  if (i >= Np) return;
+  const int hh = cell_hash(G, px[i], py[i], pz[i]);
+  hash[i] = hh;
+  atomicAdd(&count[hh], 1);
+}
+void launch_hash_count(const Geom& G, Particles P, long long Np, Bins B, hipStream_t s) {
+  hipLaunchKernelGGL(k_hash_count, dim3(nblocks(Np)), dim3(kBlock), 0, s, G, P.c[0], P.c[1],
+                     P.c[2], Np, B.hash, B.count);
+}
+
+// ------------------------------------------------------------------ scan
+// Exclusive prefix sum of the per-cell counts: offsets[c] = first sorted slot of
+// cell c, offsets[X*Y*Z] = Np. Three passes: chunk sums, scan of chunk sums in one
+// block, chunk-local scan plus chunk base.
+constexpr int kScanPerThread = 16;
+constexpr int kScanChunk = kBlock * kScanPerThread;  // 4096 cells per block
+
+__device__ int block_exclusive_scan(int v, int* total) {
+  __shared__ int wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  int base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wid) base += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return base + x - v;
+}
+
+__global__ void k_scan_reduce(const int* __restrict__ count, int n, int* __restrict__ partials) {
+  const int b = blockIdx.x;
+  const int base = b * kScanChunk + threadIdx.x * kScanPerThread;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    const int i = base + k;
+    if (i < n) s += count[i];
+  }
+  int tot;
+  block_exclusive_scan(s, &tot);
+  if (threadIdx.x == 0) partials[b] = tot;
+}
+
+__global__ void k_scan_partials(int* __restrict__ partials, int nparts, int* __restrict__ offsets,
+                                int n) {
+  const int per = (nparts + kBlock - 1) / kBlock;
+  const int b0 = threadIdx.x * per;
+  int s = 0;
+  for (int k = 0; k < per; ++k)
+    if (b0 + k < nparts) s += partials[b0 + k];
+  int tot;
+  int run = block_exclusive_scan(s, &tot);
+  for (int k = 0; k < per; ++k) {
+    if (b0 + k < nparts) {
+      const int v = partials[b0 + k];
+      partials[b0 + k] = run;
+      run += v;
+    }
+  }
+  if (threadIdx.x == 0) offsets[n] = tot;
+}
+
+__global__ void k_scan_final(const int* __restrict__ count, int n, const int* __restrict__ partials,
+                             int* __restrict__ offsets) {
+  const int b = blockIdx.x;
+  const int base = b * kScanChunk + threadIdx.x * kScanPerThread;
+  int v[kScanPerThread];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    const int i = base + k;
+    v[k] = (i < n) ? count[i] : 0;
+    s += v[k];
+  }
+  int tot;
+  int run = block_exclusive_scan(s, &tot) + partials[b];
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    const int i = base + k;
+    if (i < n) offsets[i] = run;
+    run += v[k];
+  }
+}
+
+void launch_scan(const Geom& G, Bins B, hipStream_t s) {
+  const int n = G.ncells_hash;
+  const int nparts = (n + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(kBlock), 0, s, B.count, n, B.partials);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, B.partials, nparts, B.offsets,
+                     n);
+  hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, s, B.count, n, B.partials,
+                     B.offsets);
+}
+
+// ------------------------------------------------------------------ stable counting sort
+// sort_by_key(hash, index) (Fluid/Fluid_3D_common.cuh:35), stable by previous index.
+// The atomic scatter places each particle in its cell's segment in arbitrary order
+// (and returns every count to zero); each segment is then sorted by previous
+// index, which is the stable order. applySortImpl (Fluid_3D_kernels.cuh:95-102)
+// becomes a gather of the payload channels through the permutation.
+__global__ void k_scatter(const int* __restrict__ hash, long long Np, int* __restrict__ count,
+                          const int* __restrict__ offsets, int* __restrict__ perm) {
+  const long long i = gtid();
+  if (i >= Np) return;
+  const int hh = hash[i];
+  const int r = atomicSub(&count[hh], 1) - 1;
+  perm[offsets[hh] + r] = (int)i;
+}
+
+__global__ void k_segsort(const int* __restrict__ offsets, int ncells, int* __restrict__ perm) {
+  const long long c = gtid();
+  if (c >= ncells) return;
+  const int s = offsets[c], e = offsets[c + 1];
+  for (int i = s + 1; i < e; ++i) {
+    const int v = perm[i];
+    int j = i - 1;
+    while (j >= s && perm[j] > v) {
+      perm[j + 1] = perm[j];
+      --j;
+    }
+    perm[j + 1] = v;
+  }
+}
+
+template <int K>
+__global__ void k_gather(Particles cur, Particles alt, long long Np, const int* __restrict__ perm) {
+  const long long k = gtid();
+  if (k >= Np) return;
+  const int j = perm[k];
+#pragma unroll
+  for (int ch = 0; ch < 6 + K; ++ch) alt.c[ch][k] = cur.c[ch][j];
+}
+
+void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, long long Np, int K,
+                                Bins B, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks(Np)), dim3(kBlock), 0, s, B.hash, Np, B.count,
+                     B.offsets, B.perm);
+  hipLaunchKernelGGL(k_segsort, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, B.offsets,
+                     G.ncells_hash, B.perm);
+  switch (K) {
+    case 1: hipLaunchKernelGGL(k_gather<1>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm); break;
+    case 2: hipLaunchKernelGGL(k_gather<2>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm); break;
+    case 3: hipLaunchKernelGGL(k_gather<3>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm); break;
+    default: hipLaunchKernelGGL(k_gather<4>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm); break;
+  }
+}
+
+// ------------------------------------------------------------------ P2G
+// One thread per interior cell, fusing resetCellImpl (Fluid_3D_FLIP.cu:287-302),
+// transferToCellAccumPhase (:8-69), transferToCellDividePhase (:72-116) and
+// transferVolumeFractionsToCell (:118-165). Neighbour cells are visited in the
+// reference order (dx, dy, dz from -1 to 1, flattened index range-checked in 1D)
+// and particles in sorted order, so every sum is accumulated in the same order.
+template <int K>
+__global__ void k_p2g(Geom G, Particles P, const int* __restrict__ off, Grid gr, int nv, int hv,
+                      int nf, StepParams sp) {
+  const long long n = gtid();
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell_of(G, (int)n, x, y, z);
+  const float h = G.h;
+  const float* __restrict__ px = P.c[0];
+  const float* __restrict__ py = P.c[1];
+  const float* __restrict__ pz = P.c[2];
+  const float* __restrict__ vx = P.c[3];
+  const float* __restrict__ vy = P.c[4];
+  const float* __restrict__ vz = P.c[5];
+  const float fx = (float)x, fy = (float)y, fz = (float)z;
+  const float hx = fx + 0.5f, hy = fy + 0.5f, hz = fz + 0.5f;
+  const float xp0 = fx * h, xp1 = hy * h, xp2 = hz * h;  // u face (x, y+.5, z+.5)
+  const float yp0 = hx * h, yp1 = fy * h, yp2 = hz * h;  // v face (x+.5, y, z+.5)
+  const float zp0 = hx * h, zp1 = hy * h, zp2 = fz * h;  // w face (x+.5, y+.5, z)
+  float tv0 = 0.f, tv1 = 0.f, tv2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
+  const int YZ = G.Y * G.Z;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int cell = (x + dx) * YZ + (y + dy) * G.Z + z + dz;
+        if (cell < 0 || cell >= G.ncells_hash) continue;
+        const int s = off[cell], e = off[cell + 1];
+        for (int j = s; j < e; ++j) {
+          const float qx = px[j], qy = py[j], qz = pz[j];
+          const float wx = hat3(qx - xp0, qy - xp1, qz - xp2, h);
+          const float wy = hat3(qx - yp0, qy - yp1, qz - yp2, h);
+          const float wz = hat3(qx - zp0, qy - zp1, qz - zp2, h);
+          tv0 += wx * vx[j];
+          tv1 += wy * vy[j];
+          tv2 += wz * vz[j];
+          w0 += wx;
+          w1 += wy;
+          w2 += wz;
+        }
+      }
+  uint8_t hvb = 0;
+  if (w0 > 0.f) { tv0 /= w0; hvb |= 1; }
+  if (w1 > 0.f) { tv1 /= w1; hvb |= 2; }
+  if (w2 > 0.f) { tv2 /= w2; hvb |= 4; }
+  const long long gi = G.g(x, y, z);
+  gr.vel[0][gi] = tv0;
+  gr.vel[1][gi] = tv1;
+  gr.vel[2][gi] = tv2;
+  gr.nvel[nv][0][gi] = tv0;
+  gr.nvel[nv][1][gi] = tv1;
+  gr.nvel[nv][2][gi] = tv2;
+  gr.hasvel[hv][gi] = hvb;
+  const int s0 = off[n], e0 = off[n + 1];
+  gr.content[gi] = (e0 > s0) ? FLUID : AIR;
+
+  // Volume fractions at the cell centre from the cell's own particles.
+  const float cp0 = hx * h, cp1 = hy * h, cp2 = hz * h;
+  float f[K];
+#pragma unroll
+  for (int a = 0; a < K; ++a) f[a] = 0.f;
+  float ws = 0.f;
+  for (int j = s0; j < e0; ++j) {
+    const float w = hat3(px[j] - cp0, py[j] - cp1, pz[j] - cp2, h);
+#pragma unroll
+    for (int a = 0; a < K; ++a) f[a] += w * P.c[6 + a][j];
+    ws += w;
+  }
+  if (ws > 0.f) {
+#pragma unroll
+    for (int a = 0; a < K; ++a) f[a] /= ws;
+    float sum = f[0];
+#pragma unroll
+    for (int a = 1; a < K; ++a) sum += f[a];
+#pragma unroll
+    for (int a = 0; a < K; ++a) f[a] /= sum;
+  }
+  float dens = sp.rho[0] * f[0];
+#pragma unroll
+  for (int a = 1; a < K; ++a) dens += sp.rho[a] * f[a];
+  gr.density[gi] = dens;
+#pragma unroll
+  for (int a = 0; a < K; ++a) {
+    gr.vf[a][gi] = f[a];
+    gr.nvf[nf][a][gi] = f[a];
+  }
+}
+
+void launch_p2g(const Geom& G, Particles P, long long Np, int K, Bins B, Grid gr, int nv, int hv,
+                int nf, StepParams sp, hipStream_t s) {
+  (void)Np;
+  const int nb = nblocks(G.ncells_hash);
+  switch (K) {
+    case 1: hipLaunchKernelGGL(k_p2g<1>, dim3(nb), dim3(kBlock), 0, s, G, P, B.offsets, gr, nv, hv, nf, sp); break;
+    case 2: hipLaunchKernelGGL(k_p2g<2>, dim3(nb), dim3(kBlock), 0, s, G, P, B.offsets, gr, nv, hv, nf, sp); break;
+    case 3: hipLaunchKernelGGL(k_p2g<3>, dim3(nb), dim3(kBlock), 0, s, G, P, B.offsets, gr, nv, hv, nf, sp); break;
+    default: hipLaunchKernelGGL(k_p2g<4>, dim3(nb), dim3(kBlock), 0, s, G, P, B.offsets, gr, nv, hv, nf, sp); break;
+  }
+}
+
+// ------------------------------------------------------------------ gravity + boundary
+// applyGravityImpl (Fluid/Fluid_3D_kernels.cu:4-39) gathered per target face cell,
+// then fixBoundaryX/Y/Z (Fluid_3D_kernels.cu:44-98) with the reference's pair
+// coverage (Fluid_3D_common.cu:13-41). Runs over the whole (X+1)^3 face extent,
+// since the padding layer is both a gravity target and a boundary face.
+__global__ void k_gravity_boundary(Geom G, Grid gr, int nv, StepParams sp, int doGravity,
+                                   int doBoundary) {
+  const long long n = gtid();
+  const long long nf = (long long)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
+  if (n >= nf) return;
+  int x, y, z;
+  face_of(G, (int)n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  float* __restrict__ U = gr.nvel[nv][0];
+  float* __restrict__ V = gr.nvel[nv][1];
+  float* __restrict__ W = gr.nvel[nv][2];
+  const uint8_t ct = gr.content[gi];
+  float u = U[gi], v = V[gi], w = W[gi];
+  if (doGravity) {
+    const float ax = sp.g[0] * sp.dt, ay = sp.g[1] * sp.dt, az = sp.g[2] * sp.dt;
+    const bool inter = x < G.X && y < G.Y && z < G.Z;
+    if (inter && ct == FLUID) {
+      u += ax;
+      v += ay;
+      w += az;
+    }
+    if (ct == AIR) {
+      if (sp.g[0] != 0.f && x >= 1 && x - 1 < G.X && y < G.Y && z < G.Z &&
+          gr.content[gi - G.sx] == FLUID) {
+        u += ax;
+        v += ay;
+        w += az;
+      }
+      if (sp.g[1] != 0.f && y >= 1 && x < G.X && y - 1 < G.Y && z < G.Z &&
+          gr.content[gi - G.EZ] == FLUID) {
+        u += ax;
+        v += ay;
+        w += az;
+      }
+      if (sp.g[2] != 0.f && z >= 1 && x < G.X && y < G.Y && z - 1 < G.Z &&
+          gr.content[gi - 1] == FLUID) {
+        u += ax;
+        v += ay;
+        w += az;
+      }
+    }
+  }
+  uint8_t nc = ct;
+  if (doBoundary) {
+    const bool cx = (y * (G.Z + 1) + z) < sp.coverX;
+    const bool cy = (x * (G.Z + 1) + z) < sp.coverY;
+    const bool cz = (x * (G.Y + 1) + y) < sp.coverZ;
+    if (cx && x == 0) u = fmaxf(0.f, u);
+    if (cx && x == G.X) { u = fminf(0.f, u); nc = SOLID; }
+    if (cy && y == 0) v = fmaxf(0.f, v);
+    if (cy && y == G.Y) { v = fminf(0.f, v); nc = SOLID; }
+    if (cz && z == 0) w = fmaxf(0.f, w);
+    if (cz && z == G.Z) { w = fminf(0.f, w); nc = SOLID; }
+  }
+  U[gi] = u;
+  V[gi] = v;
+  W[gi] = w;
+  if (nc != ct) gr.content[gi] = nc;
+}
+void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int doGravity,
+                             int doBoundary, hipStream_t s) {
+  const long long nf = (long long)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
+  hipLaunchKernelGGL(k_gravity_boundary, dim3(nblocks(nf)), dim3(kBlock), 0, s, G, gr, nv, sp,
+                     doGravity, doBoundary);
+}
+
+// ------------------------------------------------------------------ divergence
+// countParticlesInCellsImpl (Fluid_3D_FLIP.cu:168-185): cellEnd - cellStart with an
+// inclusive end, i.e. count-1 for occupied cells and 0 for empty ones; then
+// computeDivergenceImpl (Fluid_3D_kernels.cu:426-458) with restPPC = 8, whose
+// correction term is evaluated in double as in the reference. Also stores the
+// Jacobi right-hand side -div*density (Fluid_3D_kernels.cu:474-479).
+__global__ void k_divergence(Geom G, Grid gr, int nv, const int* __restrict__ off) {
+  const long long n = gtid();
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell_of(G, (int)n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  const float* __restrict__ U = gr.nvel[nv][0];
+  const float* __restrict__ V = gr.nvel[nv][1];
+  const float* __restrict__ W = gr.nvel[nv][2];
+  float div = V[gi + G.EZ] - V[gi] + U[gi + G.sx] - U[gi] + W[gi + 1] - W[gi];
+  const int cnt = off[n + 1] - off[n];
+  const int pc = cnt > 0 ? cnt - 1 : 0;
+  const double corr = (double)((float)pc - 8.0f) * 0.001;
+  div = (float)((double)div - corr);
+  gr.div[gi] = div;
+  gr.rhs[gi] = -div * gr.density[gi];
+}
+void launch_divergence(const Geom& G, Grid gr, int nv, Bins B, hipStream_t s) {
+  hipLaunchKernelGGL(k_divergence, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr, nv,
+                     B.offsets);
+}
+
+// ------------------------------------------------------------------ Jacobi
+__global__ void k_fill_interior_zero(Geom G, float* __restrict__ p) {
+  const long long n = gtid();
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell_of(G, (int)n, x, y, z);
+  p[G.g(x, y, z)] = 0.f;
+}
+void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_interior_zero, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G,
+                     gr.p[pb]);
+}
+
+// jacobiImpl (Fluid/Fluid_3D_kernels.cu:460-505), read-old/write-new: AIR cells 0,
+// otherwise (sum of the six clamp-to-edge neighbours + rhs) / 6, summed in the
+// reference order x+1, x-1, y+1, y-1, z+1, z-1, rhs.
+__global__ void k_jacobi(Geom G, const uint8_t* __restrict__ content, const float* __restrict__ rhs,
+                         const float* __restrict__ pin, float* __restrict__ pout) {
+  const long long n = gtid();
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell_of(G, (int)n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  if (content[gi] == AIR) {
+    pout[gi] = 0.f;
+    return;
+  }
+  const long long xp = x < G.X - 1 ? G.sx : 0, xm = x > 0 ? G.sx : 0;
+  const long long yp = y < G.Y - 1 ? G.EZ : 0, ym = y > 0 ? G.EZ : 0;
+  const long long zp = z < G.Z - 1 ? 1 : 0, zm = z > 0 ? 1 : 0;
+  float s = 0.f;
+  s += pin[gi + xp];
+  s += pin[gi - xm];
+  s += pin[gi + yp];
+  s += pin[gi - ym];
+  s += pin[gi + zp];
+  s += pin[gi - zm];
+  s += rhs[gi];
+  s /= 6.f;
+  pout[gi] = s;
+}
+void launch_jacobi(const Geom& G, Grid gr, int pin, hipStream_t s) {
+  hipLaunchKernelGGL(k_jacobi, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr.content,
+                     gr.rhs, gr.p[pin], gr.p[pin ^ 1]);
+}
+
+// diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
+// Jacobi already leaves AIR cells at 0, so this only matters with 0 pressure sweeps.
+__global__ void k_air_pressure_zero(Geom G, const uint8_t* __restrict__ content,
+                                    float* __restrict__ p) {
+  const long long n = gtid();
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell_of(G, (int)n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  if (content[gi] == AIR) p[gi] = 0.f;
+}
+void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
+  hipLaunchKernelGGL(k_air_pressure_zero, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G,
+                     gr.content, gr.p[pb]);
+}
+
+// ------------------------------------------------------------------ projection
+// updateVelocityWithPressureImpl (Fluid/Fluid_3D_kernels.cu:118-207): faces with a
+// FLUID side get u -= (p - p_other)/rho_used; hasVelocity is overwritten with
+// "face touches fluid".
+__global__ void k_project(Geom G, Grid gr, int nv, int hv, const float* __restrict__ p) {
+  const long long n = gtid();
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell_of(G, (int)n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  const uint8_t tc = gr.content[gi];
+  const float tp = p[gi];
+  const float td = gr.density[gi];
+  uint8_t hvb = 0;
+  const int coord[3] = {x, y, z};
+  const long long step[3] = {G.sx, (long long)G.EZ, 1};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (coord[d] > 0) {
+      const long long go = gi - step[d];
+      const uint8_t oc = gr.content[go];
+      const float op = p[go];
+      if (tc == FLUID || oc == FLUID) {
+        float used = td;
+        const float od = gr.density[go];
+        if (tc == FLUID && oc == FLUID)
+          used = (od + td) / 2.f;
+        else if (oc == FLUID)
+          used = od;
+        float* __restrict__ F = gr.nvel[nv][d];
+        F[gi] = F[gi] - (tp - op) / used;
+        hvb |= (uint8_t)(1u << d);
+      }
+    }
+  }
+  gr.hasvel[hv][gi] = hvb;
+}
+void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s) {
+  hipLaunchKernelGGL(k_project, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr, nv, hv,
+                     gr.p[pb]);
+}
+
+// ------------------------------------------------------------------ extrapolation
+// extrapolateVelocityByOne (Fluid/Fluid_3D_kernels.cu:210-422), read-old/write-new.
+// A face without velocity takes the mean of the same component of its six
+// neighbours that have velocity and whose velocity along the neighbour direction
+// passes the reference's epsilon test (epsilon = -1 + 1e-6). Faces outside the
+// launch domain are copied so both buffers stay complete.
+__global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
+  const long long n = gtid();
+  const long long nfc = (long long)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
+  if (n >= nfc) return;
+  int x, y, z;
+  face_of(G, (int)n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  const float* __restrict__ Ui = gr.nvel[nvin][0];
+  const float* __restrict__ Vi = gr.nvel[nvin][1];
+  const float* __restrict__ Wi = gr.nvel[nvin][2];
+  float* __restrict__ Uo = gr.nvel[nvin ^ 1][0];
+  float* __restrict__ Vo = gr.nvel[nvin ^ 1][1];
+  float* __restrict__ Wo = gr.nvel[nvin ^ 1][2];
+  const uint8_t* __restrict__ Hi = gr.hasvel[hvin];
+  uint8_t* __restrict__ Ho = gr.hasvel[hvin ^ 1];
+  if (x >= G.X || y >= G.Y || z >= G.Z) {
+    Uo[gi] = Ui[gi];
+    Vo[gi] = Vi[gi];
+    Wo[gi] = Wi[gi];
+    return;
+  }
+  const float eps = (float)(-1 + 1e-6);
+  uint8_t H = Hi[gi];
+  float val[3] = {Ui[gi], Vi[gi], Wi[gi]};
+  const float* __restrict__ F[3] = {Ui, Vi, Wi};
+  // neighbours -x, -y, -z, +x, +y, +z
+  const bool ok[6] = {x > 0, y > 0, z > 0, x < G.X - 1, y < G.Y - 1, z < G.Z - 1};
+  const long long off[6] = {-G.sx, -(long long)G.EZ, -1, G.sx, (long long)G.EZ, 1};
+  uint8_t Hn = H;
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    if (H & (1u << comp)) continue;
+    float sum = 0.f;
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (!ok[k]) continue;
+      const long long go = gi + off[k];
+      const int d = k % 3;
+      const uint8_t nh = Hi[go];
+      const float dirv = F[d][go];
+      const bool dirOk = (k < 3) ? (dirv > eps) : (dirv < -eps);
+      if ((nh & (1u << comp)) && dirOk) {
+        sum += F[comp][go];
+        cnt++;
+      }
+    }
+    if (cnt > 0) {
+      val[comp] = sum / (float)cnt;
+      Hn |= (uint8_t)(1u << comp);
+    }
+  }
+  Uo[gi] = val[0];
+  Vo[gi] = val[1];
+  Wo[gi] = val[2];
+  Ho[gi] = Hn;
+}
+void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s) {
+  const long long nfc = (long long)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
+  hipLaunchKernelGGL(k_extrapolate, dim3(nblocks(nfc)), dim3(kBlock), 0, s, G, gr, nvin, hvin);
+}
+
+// ------------------------------------------------------------------ diffusion
+// diffusionJacobiImpl (Fluid/Fluid_3D_kernels.cu:520-556), read-old/write-new:
+// f' = (lambda * sum of the six clamp-to-edge neighbours + f) / (1 + 2 lambda),
+// then normalised by its channel sum; AIR cells keep their value.
+template <int K>
+__global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda) {
+  const long long n = gtid();
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell_of(G, (int)n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  if (gr.content[gi] == AIR) {
+#pragma unroll
+    for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = gr.nvf[fin][a][gi];
+    return;
+  }
+  const long long nb[6] = {gi + (x < G.X - 1 ? G.sx : 0), gi - (x > 0 ? G.sx : 0),
+                           gi + (y < G.Y - 1 ? (long long)G.EZ : 0),
+                           gi - (y > 0 ? (long long)G.EZ : 0), gi + (z < G.Z - 1 ? 1 : 0),
+                           gi - (z > 0 ? 1 : 0)};
+  const float cc = 1.f + 2.f * lambda;
+  float r[K];
+#pragma unroll
+  for (int a = 0; a < K; ++a) r[a] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+#pragma unroll
+    for (int a = 0; a < K; ++a) r[a] += gr.nvf[fin][a][nb[k]] * lambda;
+  }
+#pragma unroll
+  for (int a = 0; a < K; ++a) r[a] += gr.nvf[fin][a][gi];
+#pragma unroll
+  for (int a = 0; a < K; ++a) r[a] /= cc;
+  float sum = r[0];
+#pragma unroll
+  for (int a = 1; a < K; ++a) sum += r[a];
+#pragma unroll
+  for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = r[a] / sum;
+}
+void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, hipStream_t s) {
+  const int nb = nblocks(G.ncells_hash);
+  switch (K) {
+    case 1: hipLaunchKernelGGL(k_diffusion<1>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda); break;
+    case 2: hipLaunchKernelGGL(k_diffusion<2>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda); break;
+    case 3: hipLaunchKernelGGL(k_diffusion<3>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda); break;
+    default: hipLaunchKernelGGL(k_diffusion<4>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda); break;
+  }
+}
+
+// ------------------------------------------------------------------ G2P and advection
+// getInterpolatedValueAtPoint (Fluid/MAC_Grid_3D.cuh:66-97) on one face component:
+// clamp to [0, N-1], trilinear lerps in the order z, y, x.
+__device__ __forceinline__ float interp(const Geom& G, const float* __restrict__ F, float x, float y,
+                                        float z) {
+  x = fmaxf(fminf(x, (float)G.X - 1.f), 0.f);
+  y = fmaxf(fminf(y, (float)G.Y - 1.f), 0.f);
+  z = fmaxf(fminf(z, (float)G.Z - 1.f), 0.f);
+  const int i = (int)floorf(x), j = (int)floorf(y), k = (int)floorf(z);
+  const float tx = x - (float)i, ty = y - (float)j, tz = z - (float)k;
+  const long long b = G.g(i, j, k);
+  const long long sx = G.sx, sy = G.EZ;
+  const float x0y0 = lerpf_(F[b], F[b + 1], tz);
+  const float x0y1 = lerpf_(F[b + sy], F[b + sy + 1], tz);
+  const float x1y0 = lerpf_(F[b + sx], F[b + sx + 1], tz);
+  const float x1y1 = lerpf_(F[b + sx + sy], F[b + sx + sy + 1], tz);
+  const float x0 = lerpf_(x0y0, x0y1, ty);
+  const float x1 = lerpf_(x1y0, x1y1, ty);
+  return lerpf_(x0, x1, tx);
+}
+// getPointVelocity / getPointNewVelocity (Fluid/MAC_Grid_3D.cuh:100-130).
+__device__ __forceinline__ void point_velocity(const Geom& G, const float* U, const float* V,
+                                               const float* W, float px, float py, float pz,
+                                               float out[3]) {
+  const float x = px / G.h, y = py / G.h, z = pz / G.h;
+  out[0] = interp(G, U, x, y - 0.5f, z - 0.5f);
+  out[1] = interp(G, V, x - 0.5f, y, z - 0.5f);
+  out[2] = interp(G, W, x - 0.5f, y - 0.5f, z);
+}
+
+// transferToParticlesImpl (Fluid/Fluid_3D_FLIP.cu:189-227). Returns newV at the
+// particle, which is also RK3's first stage.
+template <int K>
+__device__ __forceinline__ void g2p_one(const Geom& G, Particles& P, long long i, const Grid& gr,
+                                        int nv, int nf, float flip, float nvOut[3]) {
+  const float px = P.c[0][i], py = P.c[1][i], pz = P.c[2][i];
+  float nvp[3], ovp[3];
+  point_velocity(G, gr.nvel[nv][0], gr.nvel[nv][1], gr.nvel[nv][2], px, py, pz, nvp);
+  point_velocity(G, gr.vel[0], gr.vel[1], gr.vel[2], px, py, pz, ovp);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float change = nvp[c] - ovp[c];
+    const float fl = P.c[3 + c][i] + change;
+    const float pic = nvp[c];
+    P.c[3 + c][i] = flip * fl + (1.0f - flip) * pic;
+    nvOut[c] = nvp[c];
+  }
+  const int cx = clampi((int)floorf(px / G.h), 0, G.X - 1);
+  const int cy = clampi((int)floorf(py / G.h), 0, G.Y - 1);
+  const int cz = clampi((int)floorf(pz / G.h), 0, G.Z - 1);
+  const long long gc = G.g(cx, cy, cz);
+  float f[K];
+#pragma unroll
+  for (int a = 0; a < K; ++a) f[a] = gr.nvf[nf][a][gc];
+  float s = f[0] * 1.f;
+#pragma unroll
+  for (int a = 1; a < K; ++a) s += f[a] * 1.f;
+  if (s > 0.f) {
+#pragma unroll
+    for (int a = 0; a < K; ++a) f[a] /= s;
+  }
+#pragma unroll
+  for (int a = 0; a < K; ++a) P.c[6 + a][i] = f[a];
+}
+
+// moveParticlesImpl (Fluid/Fluid_3D_FLIP.cu:229-285): Ralston RK3 on newVelocity,
+// clamp to [dx/4, L - dx/4] with velocity reflection on clamp.
+__device__ __forceinline__ void advect_one(const Geom& G, Particles& P, long long i, const Grid& gr,
+                                           int nv, float dt, const float u1[3], float outPos[3]) {
+  const float b[3] = {P.c[0][i], P.c[1][i], P.c[2][i]};
+  const float* U = gr.nvel[nv][0];
+  const float* V = gr.nvel[nv][1];
+  const float* W = gr.nvel[nv][2];
+  float u2[3], u3[3], q[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) q[c] = b[c] + (dt * u1[c]) / 2.f;
+  point_velocity(G, U, V, W, q[0], q[1], q[2], u2);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) q[c] = b[c] + ((dt * u2[c]) * 3.f) / 4.f;
+  point_velocity(G, U, V, W, q[0], q[1], q[2], u3);
+  const float m = G.h / 4.f;
+  const float L[3] = {(float)G.X * G.h, (float)G.Y * G.h, (float)G.Z * G.h};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float d = b[c] + dt * ((((u1[c] * 2.f) / 9.f) + ((u2[c] * 3.f) / 9.f)) + ((u3[c] * 4.f) / 9.f));
+    if (d < m) {
+      d = m;
+      P.c[3 + c][i] *= -1.f;
+    }
+    if (d > L[c] - m) {
+      d = L[c] - m;
+      P.c[3 + c][i] *= -1.f;
+    }
+    P.c[c][i] = d;
+    outPos[c] = d;
+  }
+}
+
+template <int K>
+__global__ void k_g2p(Geom G, Particles P, long long Np, Grid gr, int nv, int nf, float flip) {
+  const long long i = gtid();
+  if (i >= Np) return;
+  float u1[3];
+  g2p_one<K>(G, P, i, gr, nv, nf, flip, u1);
+}
+
+__global__ void k_advect(Geom G, Particles P, long long Np, Grid gr, int nv, float dt) {
+  const long long i = gtid();
+  if (i >= Np) return;
+  float u1[3], pos[3];
+  point_velocity(G, gr.nvel[nv][0], gr.nvel[nv][1], gr.nvel[nv][2], P.c[0][i], P.c[1][i],
+                 P.c[2][i], u1);
+  advect_one(G, P, i, gr, nv, dt, u1, pos);
+}
+
+// Fused G2P + RK3 + hash + bin count: one pass over the particles per step.
+template <int K>
+__global__ void k_g2p_advect_hash(Geom G, Particles P, long long Np, Grid gr, int nv, int nf,
+                                  float flip, float dt, int* __restrict__ hash,
+                                  int* __restrict__ count) {
+  const long long i = gtid();
+  if (i >= Np) return;
+  float u1[3], pos[3];
+  g2p_one<K>(G, P, i, gr, nv, nf, flip, u1);
+  advect_one(G, P, i, gr, nv, dt, u1, pos);
+  const int hh = cell_hash(G, pos[0], pos[1], pos[2]);
+  hash[i] = hh;
+  atomicAdd(&count[hh], 1);
+}
+
+void launch_g2p(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv, int nf,
+                StepParams sp, hipStream_t s) {
+  const int nb = nblocks(Np);
+  switch (K) {
+    case 1: hipLaunchKernelGGL(k_g2p<1>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip); break;
+    case 2: hipLaunchKernelGGL(k_g2p<2>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip); break;
+    case 3: hipLaunchKernelGGL(k_g2p<3>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip); break;
+    default: hipLaunchKernelGGL(k_g2p<4>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip); break;
+  }
+}
+void launch_advect(const Geom& G, Particles P, long long Np, Grid gr, int nv, StepParams sp,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_advect, dim3(nblocks(Np)), dim3(kBlock), 0, s, G, P, Np, gr, nv, sp.dt);
+}
+void launch_g2p_advect_hash(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv,
+                            int nf, StepParams sp, Bins B, hipStream_t s) {
+  const int nb = nblocks(Np);
+  switch (K) {
+    case 1: hipLaunchKernelGGL(k_g2p_advect_hash<1>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.count); break;
+    case 2: hipLaunchKernelGGL(k_g2p_advect_hash<2>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.count); break;
+    case 3: hipLaunchKernelGGL(k_g2p_advect_hash<3>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.count); break;
+    default: hipLaunchKernelGGL(k_g2p_advect_hash<4>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.count); break;
+  }
+}
+
+// ------------------------------------------------------------------ FP64 reductions
+// Deterministic two-level reductions: fixed-shape block trees write per-block
+// partials, one block folds the partials in index order.
+__device__ double block_sum_d(double v) {
+  __shared__ double sh[kBlock];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Residual of the pressure system over FLUID cells: A has diagonal
+// 6 - #(out-of-domain neighbours) and -1 per FLUID neighbour, AIR neighbours are
+// 0 (Fluid/Legacy/PressureEquation.cuh:216-328); b = -div*density.
+__global__ void k_residual_partial(Geom G, const uint8_t* __restrict__ content,
+                                   const float* __restrict__ rhs, const float* __restrict__ p,
+                                   double* __restrict__ partial) {
+  double rr = 0.0, bb = 0.0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash;
+       n += stride) {
+    int x, y, z;
+    cell_of(G, (int)n, x, y, z);
+    const long long gi = G.g(x, y, z);
+    if (content[gi] != FLUID) continue;
+    const double b = (double)rhs[gi];
+    const double pc = (double)p[gi];
+    double ap = 6.0 * pc;
+    const bool in[6] = {x + 1 < G.X, x > 0, y + 1 < G.Y, y > 0, z + 1 < G.Z, z > 0};
+    const long long o[6] = {G.sx, -G.sx, (long long)G.EZ, -(long long)G.EZ, 1, -1};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (!in[k])
+        ap -= pc;
+      else if (content[gi + o[k]] == FLUID)
+        ap -= (double)p[gi + o[k]];
+    }
+    const double r = b - ap;
+    rr += r * r;
+    bb += b * b;
+  }
+  rr = block_sum_d(rr);
+  bb = block_sum_d(bb);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = rr;
+    partial[2 * blockIdx.x + 1] = bb;
+  }
+}
+
+__global__ void k_fold_partials(const double* __restrict__ partial, int nparts, int width,
+                                double* __restrict__ out) {
+  // One block; thread t folds column t % width over partial rows t/width, t/width+..
+  // then a fixed-order tree over the rows of each column.
+  __shared__ double sh[kBlock];
+  const int rows = kBlock / width;
+  const int col = threadIdx.x % width, row = threadIdx.x / width;
+  double s = 0.0;
+  if (row < rows)
+    for (int r = row; r < nparts; r += rows) s += partial[(long long)r * width + col];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if ((int)threadIdx.x < width) {
+    double t = 0.0;
+    for (int r = 0; r < rows; ++r) t += sh[r * width + threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+void launch_residual(const Geom& G, Grid gr, int pb, double* partial, int nparts, double* out,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_residual_partial, dim3(nparts), dim3(kBlock), 0, s, G, gr.content, gr.rhs,
+                     gr.p[pb], partial);
+  hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(kBlock), 0, s, partial, nparts, 2, out);
+}
+
+// Volume and mass checks: per phase the sum of particle fractions, the number of
+// FLUID cells and the particle position bounds (min stored negated so every
+// column folds with a sum or a max).
+constexpr int kStatsWidth = 16;  // f0..f3, fluid cells, -min xyz, max xyz, pad
+__global__ void k_stats_partial(Geom G, Particles P, long long Np, const uint8_t* __restrict__ content,
+                                double* __restrict__ partial) {
+  double f[4] = {0, 0, 0, 0};
+  double fluid = 0;
+  float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < Np; i += stride) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) f[a] += (double)P.c[6 + a][i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fminf(mn[a], P.c[a][i]);
+      mx[a] = fmaxf(mx[a], P.c[a][i]);
+    }
+  }
+  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash;
+       n += stride) {
+    int x, y, z;
+    cell_of(G, (int)n, x, y, z);
+    if (content[G.g(x, y, z)] == FLUID) fluid += 1.0;
+  }
+  double col[kStatsWidth];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) col[a] = block_sum_d(f[a]);
+  col[4] = block_sum_d(fluid);
+  __shared__ float shm[kBlock];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    shm[threadIdx.x] = a < 3 ? -mn[a] : mx[a - 3];
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) shm[threadIdx.x] = fmaxf(shm[threadIdx.x], shm[threadIdx.x + o]);
+      __syncthreads();
+    }
+    col[5 + a] = (double)shm[0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 11; ++a) partial[(long long)blockIdx.x * kStatsWidth + a] = col[a];
+}
+
+__global__ void k_fold_stats(const double* __restrict__ partial, int nparts, double* __restrict__ out) {
+  if (threadIdx.x >= kStatsWidth) return;
+  const int c = threadIdx.x;
+  if (c >= 11) {
+    out[c] = 0;
+    return;
+  }
+  double s = (c < 5) ? 0.0 : -1.0e300;
+  for (int r = 0; r < nparts; ++r) {
+    const double v = partial[(long long)r * kStatsWidth + c];
+    s = (c < 5) ? s + v : fmax(s, v);
+  }
+  out[c] = s;
+}
+
+void launch_stats(const Geom& G, Particles P, long long Np, Grid gr, double* partial, int nparts,
+                  double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats_partial, dim3(nparts), dim3(kBlock), 0, s, G, P, Np, gr.content,
+                     partial);
+  hipLaunchKernelGGL(k_fold_stats, dim3(1), dim3(64), 0, s, partial, nparts, out);
+}
+
+}  // namespace aqf

@@ -1,0 +1,218 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle on the same
+seeded inputs, stage by stage and over whole steps.
+
+Bar: cell indices, sort order, content flags and every integer field bit-exact;
+velocities, pressures and fractions within 1e-5 relative L2 (FP32) -- the kernels
+keep the reference's operation order and no FMA contraction, so most fields are
+expected to be bit-identical; the tests report how many values differ.
+"""
+import json
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import aquarius_amd as aq
+from aquarius_amd import configs
+from aquarius_amd.fluid import FlipHandle
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-5
+INT_FIELDS = ("content", "has_velocity", "particle_count")
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    d = np.linalg.norm(a - b)
+    return d / nb if nb > 0 else d
+
+
+def check_field(name, gpu, orc, what=""):
+    g = gpu.get_field(name)
+    o = orc.get_field(name)
+    assert g.shape == o.shape, (name, g.shape, o.shape)
+    if name in INT_FIELDS:
+        bad = int((g != o).sum())
+        assert bad == 0, f"{what}{name}: {bad} integer mismatches"
+        return 0.0
+    r = rel_l2(g, o)
+    ndiff = int((g.astype(np.float64) != o).sum())
+    log(f"  {what}{name}: rel_l2={r:.3e} differing={ndiff}/{g.size}")
+    assert np.isfinite(g).all() == np.isfinite(o).all()
+    assert r <= REL_TOL, (name, r)
+    return r
+
+
+def check_particles(gpu, orc, what=""):
+    gp, gv, gf = gpu.get_particles()
+    op, ov, of = orc.get_particles()
+    for name, g, o in (("pos", gp, op), ("vel", gv, ov), ("frac", gf, of)):
+        r = rel_l2(g, o)
+        ndiff = int((g.astype(np.float64) != o).sum())
+        log(f"  {what}particles.{name}: rel_l2={r:.3e} differing={ndiff}/{g.size}")
+        assert r <= REL_TOL, (name, r)
+
+
+def check_cells(gpu, orc, what=""):
+    gs, ge, gh, gperm = gpu.get_cells()
+    os_, oe, oh, operm = orc.get_cells()
+    assert np.array_equal(gs, os_), f"{what}cellStart differs"
+    assert np.array_equal(ge, oe), f"{what}cellEnd differs"
+    assert np.array_equal(gh, oh), f"{what}hash differs"
+    assert np.array_equal(gperm, operm), f"{what}sort permutation differs"
+
+
+STAGE_FIELDS = {
+    "g2p": [],
+    "advect": [],
+    "transfer_to_grid": ["content", "velocity", "new_velocity", "has_velocity", "density",
+                         "volume_fractions", "new_volume_fractions"],
+    "gravity": ["new_velocity"],
+    "boundary": ["new_velocity", "content"],
+    "divergence": ["divergence", "particle_count"],
+    "pressure": ["pressure"],
+    "project": ["new_velocity", "has_velocity"],
+    "extrapolate": ["new_velocity", "has_velocity"],
+    "diffusion": ["new_volume_fractions", "pressure"],
+}
+STEP_ORDER = ["g2p", "advect", "transfer_to_grid", "gravity", "boundary", "divergence", "pressure",
+              "project", "extrapolate", "diffusion"]
+
+
+def pair(cfg):
+    pos, frac = ob.seed(cfg)
+    gpu = FlipHandle(cfg, pos=pos, frac4=frac)
+    orc = ob.Oracle(cfg, pos=pos, frac4=frac)
+    return gpu, orc
+
+
+def run_stagewise(cfg, steps, label):
+    gpu, orc = pair(cfg)
+    log(f"[{label}] init, {gpu.particle_count} particles")
+    check_cells(gpu, orc, "init ")
+    check_particles(gpu, orc, "init ")
+    for f in STAGE_FIELDS["transfer_to_grid"] + ["content"]:
+        check_field(f, gpu, orc, "init ")
+    for s in range(steps):
+        for st in STEP_ORDER:
+            log(f"[{label}] step {s} stage {st}")
+            gpu.run_stage(st)
+            orc.run_stage(st)
+            if st in ("g2p", "advect", "transfer_to_grid"):
+                check_particles(gpu, orc, f"s{s} {st} ")
+            if st == "transfer_to_grid":
+                check_cells(gpu, orc, f"s{s} ")
+            for f in STAGE_FIELDS[st]:
+                check_field(f, gpu, orc, f"s{s} {st} ")
+    gpu.close()
+
+
+def test_stagewise_ball_drop_16():
+    run_stagewise(configs.ball_drop_16(), steps=3, label="ball_drop_16")
+
+
+def test_stagewise_ball_drop_64():
+    run_stagewise(configs.ball_drop_64(), steps=1, label="ball_drop_64")
+
+
+def test_stagewise_multiphase_24():
+    cfg = configs.make_config(24, dt=0.01, pressure_iters=40, diffusion_iters=10, radius=0.15,
+                              sphere_phase=1, phase_count=2, seed=21)
+    run_stagewise(cfg, steps=2, label="multiphase_24")
+
+
+def test_stagewise_edge_configs():
+    # non-cubic grid, full boundary coverage, warm-started pressure, 4 phases
+    cfg = configs.make_config(0, dt=0.01, pressure_iters=25, diffusion_iters=5, extent=(12, 16, 20),
+                              sphere_phase=3, phase_count=4, ref_boundary_coverage=0,
+                              pressure_init=aq.configs.AQ_PRESSURE_PREVIOUS, seed=5)
+    run_stagewise(cfg, steps=2, label="edge_12x16x20")
+    cfg = configs.make_config(3, dt=0.02, pressure_iters=5, diffusion_iters=2, pool_depth=0.4,
+                              radius=0.3, center=(0.5, 0.6, 0.5), seed=9)
+    run_stagewise(cfg, steps=2, label="tiny_3")
+
+
+def test_full_steps_match_oracle_16():
+    cfg = configs.ball_drop_16()
+    gpu, orc = pair(cfg)
+    log("[full steps 16] 5 steps")
+    gpu.steps(5)
+    orc.step(5)
+    check_particles(gpu, orc, "5 steps ")
+    check_cells(gpu, orc, "5 steps ")
+    for f in ("pressure", "new_velocity", "velocity", "content", "divergence"):
+        check_field(f, gpu, orc, "5 steps ")
+    assert gpu.physical_time == pytest.approx(orc.physical_time, rel=0, abs=0)
+    r_g, r_o = gpu.residual(), orc.residual()
+    log(f"  residual gpu={r_g:.6e} oracle={r_o:.6e}")
+    assert r_g == pytest.approx(r_o, rel=1e-9)
+    st = gpu.stats()
+    content = orc.get_field("content")[:16, :16, :16]
+    assert st["fluid_cells"] == int((content == 1).sum())
+    assert st["particles"] == orc.particle_count
+    _, _, frac = orc.get_particles()
+    np.testing.assert_allclose(st["phase_sum"][0], frac[:, 0].sum(), rtol=1e-12)
+    gpu.close()
+
+
+def test_fp64_error_statement_16():
+    """FP32 GPU step against the FP64 oracle run of the same step (reported)."""
+    cfg = configs.ball_drop_16()
+    pos, frac = ob.seed(cfg)
+    gpu = FlipHandle(cfg, pos=pos, frac4=frac)
+    o64 = ob.Oracle(cfg, fp64=True, pos=pos, frac4=frac)
+    gpu.step()
+    o64.step()
+    for f in ("pressure", "new_velocity"):
+        r = rel_l2(gpu.get_field(f), o64.get_field(f))
+        log(f"  fp32-gpu vs fp64-oracle {f}: rel_l2={r:.3e}")
+        assert r < 1e-3
+    gpu.close()
+
+
+def test_solve_pressure_to_tolerance_16():
+    cfg = configs.ball_drop_16()
+    gpu = FlipHandle(cfg)
+    gpu.step()
+    it, rel, ok = gpu.solve_pressure(1e-4, 20000, check_every=50)
+    log(f"  solve to 1e-4: iters={it} residual={rel:.3e} ok={ok}")
+    assert ok and rel <= 1e-4
+    # the cap is reported as a stall
+    gpu2 = FlipHandle(cfg)
+    gpu2.step()
+    it2, rel2, ok2 = gpu2.solve_pressure(1e-12, 30, check_every=10)
+    assert not ok2 and it2 == 30
+    gpu.close()
+    gpu2.close()
+
+
+def test_cpp_driver_runs():
+    from aquarius_amd.build import DRIVER
+    out = subprocess.run([str(DRIVER), "--n", "16", "--steps", "5"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    log("  driver:", rec)
+    assert rec["steps"] == 5 and rec["particles"] > 9000
+    assert rec["physical_time"] == pytest.approx(0.05, rel=1e-5)
+    assert rec["center"] == [5, 5, 5] and rec["container"] == 10
+
+
+def test_python_fluid_surface():
+    f = aq.Fluid()
+    f.init(configs.ball_drop_16())
+    for _ in range(3):
+        f.simulationStep()
+    assert f.physicalTime == pytest.approx(0.03, rel=1e-5)
+    assert f.getCenter() == (5.0, 5.0, 5.0) and f.getContainerSize() == 10.0
+    f.draw()
