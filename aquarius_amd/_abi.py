@@ -90,7 +90,45 @@ class Stats(C.Structure):
     ]
 
 
-# name -> (restype, argtypes); every symbol declared in include/aqflip.h
+class UNetConfig(C.Structure):
+    _fields_ = [
+        ("channels", C.c_int32 * 3),
+        ("seed", C.c_uint64),
+        ("lr", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("eps", C.c_float),
+        ("target_scale", C.c_float),
+    ]
+
+
+class Solver(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("tol", C.c_double),
+        ("max_iters", C.c_int32),
+        ("check_every", C.c_int32),
+        ("stall_checks", C.c_int32),
+    ]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [
+        ("solves", C.c_int64),
+        ("sweeps", C.c_int64),
+        ("fallbacks", C.c_int64),
+        ("capped", C.c_int64),
+        ("last_sweeps", C.c_int32),
+        ("last_residual", C.c_double),
+        ("last_guess_residual", C.c_double),
+        ("last_fallback", C.c_int32),
+    ]
+
+
+AQ_SOLVE_FIXED, AQ_SOLVE_TOL_ZERO, AQ_SOLVE_TOL_NN = 0, 1, 2
+
+# name -> (restype, argtypes); every symbol declared in include/aqflip.h and
+# include/aqflip_unet.h
 _P = C.c_void_p
 _FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int32)
@@ -121,6 +159,25 @@ PROTOTYPES = {
     "aq_flip_container_size": (C.c_float, [_P]),
     "aq_last_error": (C.c_char_p, []),
     "aq_flip_version": (C.c_char_p, []),
+    # include/aqflip_unet.h
+    "aq_unet_config_default": (C.c_int, [C.POINTER(UNetConfig)]),
+    "aq_unet_create": (C.c_int, [C.POINTER(UNetConfig), C.POINTER(_P)]),
+    "aq_unet_destroy": (None, [_P]),
+    "aq_unet_param_count": (C.c_int64, [_P]),
+    "aq_unet_get_params": (C.c_int, [_P, _FP]),
+    "aq_unet_set_params": (C.c_int, [_P, _FP]),
+    "aq_unet_forward_host": (C.c_int, [_P, _FP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _FP]),
+    "aq_unet_gradients_host": (C.c_int, [_P, _FP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                         C.POINTER(C.c_double), _FP]),
+    "aq_unet_dataset_reserve": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "aq_unet_dataset_add_flip": (C.c_int, [_P, _P]),
+    "aq_unet_dataset_size": (C.c_int32, [_P]),
+    "aq_unet_train": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _FP]),
+    "aq_unet_eval": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "aq_flip_nn_guess": (C.c_int, [_P, _P]),
+    "aq_flip_set_solver": (C.c_int, [_P, C.POINTER(Solver), _P]),
+    "aq_flip_solver_stats_get": (C.c_int, [_P, C.POINTER(SolverStats), C.c_int32]),
+    "aq_unet_mfma_probe": (C.c_int, [_FP, _FP, _FP]),
 }
 
 _lib = None

@@ -36,7 +36,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-LIB_SOURCES = ["flip_kernels.hip", "flip_api.cpp", "flip_seed.cpp"]
+LIB_SOURCES = ["flip_kernels.hip", "flip_api.cpp", "flip_seed.cpp", "unet_kernels.hip", "unet_api.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

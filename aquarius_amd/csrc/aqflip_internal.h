@@ -1,10 +1,11 @@
 // aqflip_internal.h -- device data layout and kernel launchers of libaqflip.
 //
 // HBM layout (one GPU, everything resident for the life of the handle):
-//   * Particles: structure of arrays, 10 float channels of Np each
-//     (px py pz vx vy vz f0 f1 f2 f3), two buffers (current and sorted copy).
-//     Only the first K fraction channels are read or written; K is the number of
-//     phases in use, the rest stay exactly zero.
+//   * Particles: 16-byte records (px,py,pz,vx)(vy,vz,f0,f1)[(f2,f3,-,-)], i.e. 32
+//     bytes per particle with up to 2 phases (48 with 3-4), two buffers (current
+//     and sorted copy). Every particle kernel moves whole records with 16-byte
+//     loads; only the first K fraction channels are used (K = phases in use), the
+//     rest stay exactly zero.
 //   * Grid: every field uses one padded cell index over the face extent
 //     (X+1)x(Y+1)x(Z+1) of MAC_Grid_3D (Fluid/MAC_Grid_3D.cu:53-71):
 //     g(x,y,z) = (x*EY + y)*EZ + z with EY = Y+1 and EZ = Z+1 rounded up to 8, so
@@ -13,7 +14,8 @@
 //     interior x<X, y<Y, z<Z only. Face velocities are split per component
 //     (u, v, w) so each stencil reads one dense array.
 //   * Cell bins: counts, exclusive offsets (X*Y*Z + 1) over the particle hash
-//     x*Y*Z + y*Z + z (Fluid/Fluid_3D_kernels.cuh:8-27), and the permutation.
+//     x*Y*Z + y*Z + z (Fluid/Fluid_3D_kernels.cuh:8-27), per-particle hash and
+//     in-cell rank, and the sort permutation.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,21 +25,52 @@ namespace aqf {
 
 enum : uint8_t { AIR = 0, FLUID = 1, SOLID = 2 };  // Fluid/MAC_Grid_3D.cuh:15-17
 
+// Division by a runtime constant d (n < 2^31): q = (umulhi(n, m) + n) >> s with
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1.
+struct FastDiv {
+  uint32_t d, m, s;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  const uint64_t m = ((1ull << 32) * ((1ull << s) - d)) / d + 1;
+  return FastDiv{d, (uint32_t)m, s};
+}
+__host__ __device__ inline uint32_t fast_div(uint32_t n, FastDiv f) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const uint32_t t = __umulhi(n, f.m);
+#else
+  const uint32_t t = (uint32_t)(((uint64_t)n * f.m) >> 32);
+#endif
+  return (t + n) >> f.s;
+}
+
 struct Geom {
   int X, Y, Z;        // cells
   int EY, EZ;         // padded strides: EY = Y+1, EZ = round_up(Z+1, 8)
   long long sx;       // x stride = (long long)EY*EZ
   long long ncell;    // (X+1)*EY*EZ
   int ncells_hash;    // X*Y*Z
+  int nfaces;         // (X+1)*(Y+1)*(Z+1)
   float h;            // cellPhysicalSize
+  double inv_h;       // 1/h rounded to double (see div_exact)
+  FastDiv divZ, divYZ, divZ1, divYZ1;  // Z, Y*Z, Z+1, (Y+1)*(Z+1)
   __host__ __device__ long long g(int x, int y, int z) const {
     return (long long)x * sx + (long long)y * EZ + z;
   }
 };
 
+// Particle state as 16-byte records, in the current (sorted) order:
+//   a = (px, py, pz, vx), b = (vy, vz, f0, f1), c = (f2, f3, 0, 0) (c only with K > 2).
 struct Particles {
-  float* c[10];  // px py pz vx vy vz f0 f1 f2 f3
+  float4* a;
+  float4* b;
+  float4* c;
 };
+
+// Bit pattern of the Jacobi right-hand side of an AIR cell: a NaN payload that
+// arithmetic never produces (AMD and x86 return the canonical 0x7FC00000).
+constexpr uint32_t kAirRhsBits = 0x7FA1A1A1u;
 
 struct Grid {
   uint8_t* content;     // AIR/FLUID/SOLID
@@ -46,17 +79,18 @@ struct Grid {
   uint8_t* hasvel[2];   // hasVelocity bits 0..2, ping-pong
   float* p[2];          // pressure, ping-pong
   float* div;           // divergence
-  float* rhs;           // -divergence*density (Jacobi right-hand side)
+  float* rhs;           // -divergence*density (Jacobi right-hand side); kAirRhsBits on AIR
   float* density;
   float* vf[4];         // volumeFractions
   float* nvf[2][4];     // newVolumeFractions, ping-pong
 };
 
 struct Bins {
-  int* count;     // X*Y*Z per-cell counts (returns to zero after the scatter)
+  int* count;     // X*Y*Z per-cell counts
   int* offsets;   // X*Y*Z + 1 exclusive prefix
   int* partials;  // scan block partials
   int* hash;      // per particle, previous order
+  int* rank;      // per particle: slot within its cell's segment (pre-sort)
   int* perm;      // sorted slot -> previous index
 };
 
@@ -65,6 +99,7 @@ struct StepParams {
   float g[3];
   float rho[4];
   float lambda;  // D*dt/dx^2
+  double inv_cc; // 1/(1 + 2*lambda) rounded to double
   int coverX, coverY, coverZ;  // fixBoundary pair coverage
 };
 
@@ -79,12 +114,14 @@ void launch_p2g(const Geom& G, Particles P, long long Np, int K, Bins B, Grid gr
 void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int doGravity,
                              int doBoundary, hipStream_t s);
 void launch_divergence(const Geom& G, Grid gr, int nv, Bins B, hipStream_t s);
+void launch_refresh_rhs(const Geom& G, Grid gr, hipStream_t s);
 void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 void launch_jacobi(const Geom& G, Grid gr, int pin, hipStream_t s);
 void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s);
 void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s);
-void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, hipStream_t s);
+void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc,
+                      hipStream_t s);
 void launch_g2p(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv, int nf,
                 StepParams sp, hipStream_t s);
 void launch_advect(const Geom& G, Particles P, long long Np, Grid gr, int nv, StepParams sp,
@@ -94,7 +131,7 @@ void launch_g2p_advect_hash(const Geom& G, Particles P, long long Np, int K, Gri
 // FP64 reductions: residual terms (sum r^2, sum b^2) and volume statistics.
 void launch_residual(const Geom& G, Grid gr, int pb, double* partial, int nparts, double* out,
                      hipStream_t s);
-void launch_stats(const Geom& G, Particles P, long long Np, Grid gr, double* partial, int nparts,
-                  double* out, hipStream_t s);
+void launch_stats(const Geom& G, Particles P, long long Np, int K, Grid gr, double* partial,
+                  int nparts, double* out, hipStream_t s);
 
 }  // namespace aqf

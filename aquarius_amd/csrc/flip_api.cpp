@@ -10,12 +10,15 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/aqflip.h"
+#include "../../include/aqflip_unet.h"
 #include "aqflip_internal.h"
+#include "flip_view.h"
 
 namespace aqf {
 int64_t seed_scene(const aq_flip_config& cfg, std::vector<float>* pos, std::vector<float>* frac);
@@ -77,6 +80,11 @@ struct aq_flip {
   double* red_out = nullptr;
   float physicalTime = 0.f;
 
+  // Pressure solve mode (fixed sweeps as the reference, or to a tolerance).
+  aq_flip_solver solver{AQ_SOLVE_FIXED, 1e-4, 100000, 50, 20};
+  aq_unet* net = nullptr;
+  aq_flip_solver_stats sstats{};
+
   bool timing = false;
   double stage_ms[AQ_STAGE_COUNT]{};
   int64_t stage_calls[AQ_STAGE_COUNT]{};
@@ -86,6 +94,16 @@ struct aq_flip {
     int stage;
     hipEvent_t a, b;
   };
+  // Captured steps: one executable graph per ping-pong state at step entry (odd
+  // pass counts alternate buffers between steps), with the state it leaves.
+  struct StepGraph {
+    int key[5];
+    int next[5];
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<StepGraph> graphs;
+  bool use_graphs = true;
   std::vector<hipEvent_t> event_pool;
   std::vector<Mark> pending;
   size_t pool_used = 0;
@@ -93,6 +111,10 @@ struct aq_flip {
 
   ~aq_flip() {
     if (stream) hipStreamSynchronize(stream);
+    for (StepGraph& g : graphs) {
+      hipGraphExecDestroy(g.exec);
+      hipGraphDestroy(g.graph);
+    }
     for (void* p : allocs) hipFree(p);
     for (hipEvent_t e : event_pool) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
@@ -208,7 +230,13 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   G.sx = (long long)G.EY * G.EZ;
   G.ncell = (long long)(G.X + 1) * G.sx;
   G.ncells_hash = G.X * G.Y * G.Z;
+  G.nfaces = (G.X + 1) * (G.Y + 1) * (G.Z + 1);
   G.h = cfg->grid_physical_size / (float)G.Y;  // Fluid_3D_FLIP.cu:444
+  G.inv_h = 1.0 / (double)G.h;
+  G.divZ = make_fastdiv((uint32_t)G.Z);
+  G.divYZ = make_fastdiv((uint32_t)(G.Y * G.Z));
+  G.divZ1 = make_fastdiv((uint32_t)(G.Z + 1));
+  G.divYZ1 = make_fastdiv((uint32_t)((G.Y + 1) * (G.Z + 1)));
   h->Np = Np;
 
   StepParams& sp = h->sp;
@@ -217,14 +245,18 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   for (int a = 0; a < 3; ++a) sp.g[a] = cfg->gravity[a];
   for (int a = 0; a < 4; ++a) sp.rho[a] = cfg->phase_densities[a];
   sp.lambda = (cfg->diffusion_coeff * cfg->dt) / (G.h * G.h);  // Fluid_3D_common.cu:86
+  sp.inv_cc = 1.0 / (double)(1.f + 2.f * sp.lambda);             // Fluid_3D_kernels.cu:538
   const bool ref = cfg->ref_boundary_coverage != 0;
   sp.coverX = boundary_coverage(G.Y, G.Z, ref);
   sp.coverY = boundary_coverage(G.X, G.Z, ref);
   sp.coverZ = boundary_coverage(G.X, G.Y, ref);
 
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  for (int b = 0; b < 2; ++b)
-    for (int ch = 0; ch < 10; ++ch) AQ_TRY(h->allocT(&h->P[b].c[ch], (size_t)Np));
+  for (int b = 0; b < 2; ++b) {
+    AQ_TRY(h->allocT(&h->P[b].a, (size_t)Np));
+    AQ_TRY(h->allocT(&h->P[b].b, (size_t)Np));
+    AQ_TRY(h->allocT(&h->P[b].c, (size_t)Np));
+  }
   const size_t nc = (size_t)G.ncell;
   Grid& gr = h->gr;
   AQ_TRY(h->allocT(&gr.content, nc));
@@ -250,6 +282,7 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   AQ_TRY(h->allocT(&h->B.offsets, (size_t)nch + 1));
   AQ_TRY(h->allocT(&h->B.partials, (size_t)(nch + 4095) / 4096 + 1));
   AQ_TRY(h->allocT(&h->B.hash, (size_t)Np));
+  AQ_TRY(h->allocT(&h->B.rank, (size_t)Np));
   AQ_TRY(h->allocT(&h->B.perm, (size_t)Np));
   h->nparts_red = 512;
   AQ_TRY(h->allocT(&h->red_partial, (size_t)h->nparts_red * 16));
@@ -259,21 +292,23 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
 
 aq_status upload_particles(aq_flip* h, const float* pos, const float* vel, const float* frac4) {
   const long long n = h->Np;
-  std::vector<float> chan((size_t)n);
-  Particles& P = h->P[h->cur];
-  for (int a = 0; a < 3; ++a) {
-    for (long long i = 0; i < n; ++i) chan[i] = pos[3 * i + a];
-    HIP_TRY(hipMemcpyAsync(P.c[a], chan.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    for (long long i = 0; i < n; ++i) chan[i] = vel ? vel[3 * i + a] : 0.f;
-    HIP_TRY(hipMemcpyAsync(P.c[3 + a], chan.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+  std::vector<float4> a((size_t)n), b((size_t)n), c((size_t)n);
+  for (long long i = 0; i < n; ++i) {
+    const float* p = pos + 3 * i;
+    float v[3] = {0.f, 0.f, 0.f}, f[4] = {0.f, 0.f, 0.f, 0.f};
+    if (vel)
+      for (int k = 0; k < 3; ++k) v[k] = vel[3 * i + k];
+    if (frac4)
+      for (int k = 0; k < 4; ++k) f[k] = frac4[4 * i + k];
+    a[i] = make_float4(p[0], p[1], p[2], v[0]);
+    b[i] = make_float4(v[1], v[2], f[0], f[1]);
+    c[i] = make_float4(f[2], f[3], 0.f, 0.f);
   }
-  for (int a = 0; a < 4; ++a) {
-    for (long long i = 0; i < n; ++i) chan[i] = frac4 ? frac4[4 * i + a] : 0.f;
-    HIP_TRY(hipMemcpyAsync(P.c[6 + a], chan.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-  }
+  const Particles& P = h->P[h->cur];
+  HIP_TRY(hipMemcpyAsync(P.a, a.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(P.b, b.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(P.c, c.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return AQ_OK;
 }
 
@@ -324,7 +359,7 @@ void stage_diffusion(aq_flip* h, bool inStep) {
   if (!inStep || h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
   if (inStep && h->K == 1) return;
   for (int i = 0; i < iters; ++i) {
-    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->stream);
+    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, h->stream);
     h->nf ^= 1;
   }
 }
@@ -334,7 +369,10 @@ aq_status run_stage(aq_flip* h, int st) {
   switch (st) {
     case AQ_STAGE_G2P: launch_g2p(h->G, h->P[h->cur], h->Np, h->K, h->gr, h->nv, h->nf, h->sp, h->stream); break;
     case AQ_STAGE_ADVECT: launch_advect(h->G, h->P[h->cur], h->Np, h->gr, h->nv, h->sp, h->stream); break;
-    case AQ_STAGE_TRANSFER_TO_GRID: stage_transfer(h); break;
+    case AQ_STAGE_TRANSFER_TO_GRID:
+      stage_transfer(h);
+      launch_refresh_rhs(h->G, h->gr, h->stream);  // content changed: re-mark AIR for Jacobi
+      break;
     case AQ_STAGE_GRAVITY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 1, 0, h->stream); break;
     case AQ_STAGE_BOUNDARY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 0, 1, h->stream); break;
     case AQ_STAGE_DIVERGENCE: launch_divergence(h->G, h->gr, h->nv, h->B, h->stream); break;
@@ -346,6 +384,94 @@ aq_status run_stage(aq_flip* h, int st) {
   }
   HIP_TRY(hipGetLastError());
   h->end(st);
+  return AQ_OK;
+}
+
+aq_status residual_now(aq_flip* h, double* rel) {
+  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  HIP_TRY(hipGetLastError());
+  double o[2];
+  HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  *rel = o[1] > 0 ? std::sqrt(o[0]) / std::sqrt(o[1]) : std::sqrt(o[0]);
+  return AQ_OK;
+}
+
+// Jacobi sweeps from the current pressure until the FP64 relative residual is
+// <= tol (checked every check_every sweeps) or `budget` sweeps are spent. With
+// detect_stall, `stall_checks` consecutive checks without a 1% improvement over
+// the best residual (or a non-finite residual) end the solve as stalled.
+aq_status sweep_to_tol(aq_flip* h, int budget, bool detect_stall, int* sweeps, double* res,
+                       bool* stalled) {
+  const aq_flip_solver& S = h->solver;
+  *sweeps = 0;
+  *stalled = false;
+  AQ_TRY(residual_now(h, res));
+  double best = *res;
+  int flat = 0;
+  while (!(*res <= S.tol) && *sweeps < budget) {
+    if (!std::isfinite(*res)) {
+      *stalled = true;
+      break;
+    }
+    const int k = std::min(S.check_every, budget - *sweeps);
+    jacobi_sweeps(h, k);
+    *sweeps += k;
+    AQ_TRY(residual_now(h, res));
+    if (detect_stall) {
+      if (std::isfinite(*res) && *res < 0.99 * best) {
+        best = *res;
+        flat = 0;
+      } else if (++flat >= S.stall_checks || !std::isfinite(*res)) {
+        *stalled = true;
+        break;
+      }
+    }
+  }
+  return AQ_OK;
+}
+
+// The pressure solve of one step, per the selected mode.
+aq_status pressure_stage(aq_flip* h) {
+  aq_flip_solver_stats& st = h->sstats;
+  if (h->solver.mode == AQ_SOLVE_FIXED) {
+    if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
+    h->begin(AQ_STAGE_PRESSURE);  // the Jacobi sweeps only
+    jacobi_sweeps(h, h->cfg.pressure_iters);
+    h->end(AQ_STAGE_PRESSURE);
+    return AQ_OK;
+  }
+  h->begin(AQ_STAGE_PRESSURE);
+  int sweeps = 0, total = 0;
+  double res = 0, guess = 1.0;
+  bool stalled = false, fallback = false;
+  if (h->solver.mode == AQ_SOLVE_TOL_NN) {
+    if (!h->net) return fail(AQ_EINVAL, "AQ_SOLVE_TOL_NN needs a U-Net");
+    AQ_TRY(aq_flip_nn_guess(h, h->net));
+    AQ_TRY(residual_now(h, &guess));
+    // A guess that is non-finite or no better than zero (relative residual 1) is rejected.
+    if (!std::isfinite(guess) || guess >= 1.0) fallback = true;
+    if (!fallback) {
+      AQ_TRY(sweep_to_tol(h, h->solver.max_iters, true, &sweeps, &res, &stalled));
+      total += sweeps;
+      if (stalled && !(res <= h->solver.tol)) fallback = true;
+    }
+  }
+  if (h->solver.mode == AQ_SOLVE_TOL_ZERO || fallback) {
+    launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
+    AQ_TRY(sweep_to_tol(h, std::max(0, h->solver.max_iters), false, &sweeps, &res, &stalled));
+    total += sweeps;
+    if (h->solver.mode == AQ_SOLVE_TOL_ZERO) guess = 1.0;
+  }
+  h->end(AQ_STAGE_PRESSURE);
+  st.solves += 1;
+  st.sweeps += total;
+  st.fallbacks += fallback ? 1 : 0;
+  st.capped += (res <= h->solver.tol) ? 0 : 1;
+  st.last_sweeps = total;
+  st.last_residual = res;
+  st.last_guess_residual = guess;
+  st.last_fallback = fallback ? 1 : 0;
   return AQ_OK;
 }
 
@@ -365,10 +491,7 @@ aq_status enqueue_step(aq_flip* h) {
   h->begin(AQ_STAGE_DIVERGENCE);
   launch_divergence(h->G, h->gr, h->nv, h->B, h->stream);
   h->end(AQ_STAGE_DIVERGENCE);
-  if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
-  h->begin(AQ_STAGE_PRESSURE);  // the Jacobi sweeps only
-  jacobi_sweeps(h, h->cfg.pressure_iters);
-  h->end(AQ_STAGE_PRESSURE);
+  AQ_TRY(pressure_stage(h));
   h->begin(AQ_STAGE_PROJECT);
   launch_project(h->G, h->gr, h->nv, h->hv, h->pb, h->stream);
   h->end(AQ_STAGE_PROJECT);
@@ -383,9 +506,56 @@ aq_status enqueue_step(aq_flip* h) {
   return AQ_OK;
 }
 
+void drop_graphs(aq_flip* h) {
+  hipStreamSynchronize(h->stream);
+  for (auto& g : h->graphs) {
+    hipGraphExecDestroy(g.exec);
+    hipGraphDestroy(g.graph);
+  }
+  h->graphs.clear();
+}
+
+void state_of(const aq_flip* h, int k[5]) {
+  k[0] = h->cur, k[1] = h->nv, k[2] = h->hv, k[3] = h->pb, k[4] = h->nf;
+}
+
+// One step through a captured graph (eager launches when timing is on, so the
+// stage events sit on the stream between kernels).
+aq_status graph_step(aq_flip* h) {
+  if (h->timing || !h->use_graphs || h->solver.mode != AQ_SOLVE_FIXED) return enqueue_step(h);
+  int key[5];
+  state_of(h, key);
+  for (auto& g : h->graphs) {
+    if (std::memcmp(g.key, key, sizeof(key)) == 0) {
+      HIP_TRY(hipGraphLaunch(g.exec, h->stream));
+      h->cur = g.next[0], h->nv = g.next[1], h->hv = g.next[2], h->pb = g.next[3], h->nf = g.next[4];
+      h->physicalTime += h->cfg.dt;
+      return AQ_OK;
+    }
+  }
+  aq_flip::StepGraph g{};
+  std::memcpy(g.key, key, sizeof(key));
+  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  aq_status st = enqueue_step(h);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  if (st != AQ_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) return fail(AQ_EHIP, std::string("step capture: ") + hipGetErrorString(e));
+  g.graph = graph;
+  HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+  state_of(h, g.next);
+  h->graphs.push_back(g);
+  HIP_TRY(hipGraphLaunch(g.exec, h->stream));
+  return AQ_OK;
+}
+
 aq_status create_common(const aq_flip_config* cfg, const float* pos, const float* vel,
                         const float* frac4, long long n, aq_flip** out) {
   aq_flip* h = new aq_flip;
+  if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
   aq_status st = setup(h, cfg, n);
   if (st == AQ_OK) st = check_positions(h, pos, n);
   if (st == AQ_OK) {
@@ -396,6 +566,7 @@ aq_status create_common(const aq_flip_config* cfg, const float* pos, const float
     // Fluid::init: fixBoundary, then transferToGrid (Fluid_3D_FLIP.cu:485,500).
     launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 0, 1, h->stream);
     stage_transfer(h);
+    launch_refresh_rhs(h->G, h->gr, h->stream);
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) st = fail(AQ_EHIP, std::string("init: ") + hipGetErrorString(e));
@@ -444,7 +615,32 @@ aq_status h2d(aq_flip* h, void* dst, const std::vector<T>& src) {
 
 }  // namespace
 
+namespace aqf {
+FlipView flip_view(aq_flip* h) { return FlipView{h->G, &h->gr, &h->pb, h->stream}; }
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace aqf
+
 extern "C" {
+
+aq_status aq_flip_set_solver(aq_flip* h, const aq_flip_solver* s, aq_unet* net) {
+  if (!h || !s) return fail(AQ_EINVAL, "null argument");
+  if (s->mode < AQ_SOLVE_FIXED || s->mode > AQ_SOLVE_TOL_NN) return fail(AQ_EINVAL, "unknown solver mode");
+  if (s->mode != AQ_SOLVE_FIXED && (!(s->tol > 0) || s->max_iters < 0 || s->check_every < 1 || s->stall_checks < 1))
+    return fail(AQ_EINVAL, "tolerance solves need tol > 0, max_iters >= 0, check_every >= 1, stall_checks >= 1");
+  if (s->mode == AQ_SOLVE_TOL_NN && !net) return fail(AQ_EINVAL, "AQ_SOLVE_TOL_NN needs a U-Net");
+  if (s->mode == AQ_SOLVE_TOL_NN && (h->G.X % 4 || h->G.Y % 4 || h->G.Z % 4))
+    return fail(AQ_EINVAL, "the U-Net needs grid extents that are multiples of 4");
+  h->solver = *s;
+  h->net = net;
+  return AQ_OK;
+}
+
+aq_status aq_flip_solver_stats_get(aq_flip* h, aq_flip_solver_stats* out, int32_t reset) {
+  if (!h || !out) return fail(AQ_EINVAL, "null argument");
+  *out = h->sstats;
+  if (reset) h->sstats = aq_flip_solver_stats{};
+  return AQ_OK;
+}
 
 aq_status aq_flip_config_default(aq_flip_config* c) {
   if (!c) return fail(AQ_EINVAL, "null config");
@@ -524,14 +720,14 @@ void aq_flip_destroy(aq_flip* h) { delete h; }
 
 aq_status aq_flip_step(aq_flip* h) {
   if (!h) return fail(AQ_EINVAL, "null handle");
-  AQ_TRY(enqueue_step(h));
+  AQ_TRY(graph_step(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return h->collect();
 }
 
 aq_status aq_flip_steps(aq_flip* h, int32_t count) {
   if (!h || count < 0) return fail(AQ_EINVAL, "bad argument");
-  for (int i = 0; i < count; ++i) AQ_TRY(enqueue_step(h));
+  for (int i = 0; i < count; ++i) AQ_TRY(graph_step(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
   return h->collect();
 }
@@ -549,18 +745,28 @@ int64_t aq_flip_particle_count(const aq_flip* h) { return h ? h->Np : 0; }
 aq_status aq_flip_get_particles(aq_flip* h, float* pos, float* vel, float* frac4) {
   if (!h) return fail(AQ_EINVAL, "null handle");
   const long long n = h->Np;
-  std::vector<float> ch;
+  std::vector<float4> a, b, c;
   const Particles& P = h->P[h->cur];
-  for (int c = 0; c < 10; ++c) {
-    float* dst = c < 3 ? pos : (c < 6 ? vel : frac4);
-    if (!dst) continue;
-    const int w = c < 6 ? 3 : 4, a = c < 3 ? c : (c < 6 ? c - 3 : c - 6);
-    if (c >= 6 && a >= h->K) {
-      for (long long i = 0; i < n; ++i) dst[w * i + a] = 0.f;
-      continue;
+  AQ_TRY(d2h(h, a, P.a, (size_t)n));
+  AQ_TRY(d2h(h, b, P.b, (size_t)n));
+  if (h->K > 2) AQ_TRY(d2h(h, c, P.c, (size_t)n));
+  for (long long i = 0; i < n; ++i) {
+    if (pos) {
+      pos[3 * i] = a[i].x;
+      pos[3 * i + 1] = a[i].y;
+      pos[3 * i + 2] = a[i].z;
     }
-    AQ_TRY(d2h(h, ch, P.c[c], (size_t)n));
-    for (long long i = 0; i < n; ++i) dst[w * i + a] = ch[i];
+    if (vel) {
+      vel[3 * i] = a[i].w;
+      vel[3 * i + 1] = b[i].x;
+      vel[3 * i + 2] = b[i].y;
+    }
+    if (frac4) {
+      frac4[4 * i] = b[i].z;
+      frac4[4 * i + 1] = h->K > 1 ? b[i].w : 0.f;
+      frac4[4 * i + 2] = h->K > 2 ? c[i].x : 0.f;
+      frac4[4 * i + 3] = h->K > 3 ? c[i].y : 0.f;
+    }
   }
   return AQ_OK;
 }
@@ -570,7 +776,9 @@ aq_status aq_flip_set_particles(aq_flip* h, const float* pos, const float* vel, 
   if (!h || !pos) return fail(AQ_EINVAL, "null argument");
   if (n != h->Np) return fail(AQ_EINVAL, "particle count differs from the handle's");
   AQ_TRY(check_positions(h, pos, n));
-  h->K = std::max(h->K, channels_for(h->cfg, frac4, n));
+  const int K = std::max(h->K, channels_for(h->cfg, frac4, n));
+  if (K != h->K) drop_graphs(h);
+  h->K = K;
   return upload_particles(h, pos, vel, frac4);
 }
 
@@ -691,6 +899,8 @@ aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t b
       AQ_TRY(d2h(h, c, h->gr.content, nc));
       each([&](size_t o, size_t g) { c[g] = (uint8_t)I[o]; });
       AQ_TRY(h2d(h, h->gr.content, c));
+      launch_refresh_rhs(h->G, h->gr, h->stream);
+      HIP_TRY(hipStreamSynchronize(h->stream));
       break;
     }
     case AQ_FIELD_VELOCITY:
@@ -716,11 +926,8 @@ aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t b
       AQ_TRY(h2d(h, d, v));
       if (field != AQ_FIELD_PRESSURE) {
         // keep the Jacobi right-hand side consistent with divergence*density
-        std::vector<float> dv, de;
-        AQ_TRY(d2h(h, dv, h->gr.div, nc));
-        AQ_TRY(d2h(h, de, h->gr.density, nc));
-        for (size_t i = 0; i < nc; ++i) dv[i] = -dv[i] * de[i];
-        AQ_TRY(h2d(h, h->gr.rhs, dv));
+        launch_refresh_rhs(h->G, h->gr, h->stream);
+        HIP_TRY(hipStreamSynchronize(h->stream));
       }
       break;
     }
@@ -742,7 +949,10 @@ aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t b
             need = a + 1;
             break;
           }
-      h->K = std::max(h->K, need);
+      if (need > h->K) {
+        drop_graphs(h);
+        h->K = need;
+      }
       for (int a = 0; a < h->K; ++a) {
         float* d = field == AQ_FIELD_VOLUME_FRACTIONS ? h->gr.vf[a] : h->gr.nvf[h->nf][a];
         std::vector<float> v;
@@ -800,7 +1010,7 @@ aq_status aq_flip_residual_fp64(aq_flip* h, double* rel) {
 
 aq_status aq_flip_volume_stats(aq_flip* h, aq_flip_stats* out) {
   if (!h || !out) return fail(AQ_EINVAL, "null argument");
-  launch_stats(h->G, h->P[h->cur], h->Np, h->gr, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  launch_stats(h->G, h->P[h->cur], h->Np, h->K, h->gr, h->red_partial, h->nparts_red, h->red_out, h->stream);
   HIP_TRY(hipGetLastError());
   double o[16];
   HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));

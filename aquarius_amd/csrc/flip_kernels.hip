@@ -20,43 +20,48 @@ constexpr int kXCD = 8;
 // hardware block id so that each XCD walks one contiguous range of logical
 // blocks: a slab of cells (or a run of sorted particles) then stays in one XCD's
 // L2 for the whole launch and across consecutive launches of the same shape.
-__device__ __forceinline__ long long xcd_block() {
+__device__ __forceinline__ int xcd_block() {
   const int nb = gridDim.x, bid = blockIdx.x;
   const int per = nb / kXCD, rem = nb % kXCD;
   const int xcd = bid % kXCD, k = bid / kXCD;
-  return (long long)xcd * per + min(xcd, rem) + k;
+  return xcd * per + min(xcd, rem) + k;
 }
-__device__ __forceinline__ long long gtid() { return xcd_block() * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int gtid() { return xcd_block() * kBlock + (int)threadIdx.x; }
 
 static inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // ------------------------------------------------------------------ helpers
-// trilinearUnitKernel / trilinearHatKernel(float3) (Fluid/WeightKernels.cuh:131-146).
+// trilinearUnitKernel (Fluid/WeightKernels.cuh:131-136).
 __device__ __forceinline__ float hat_unit(float r) {
   r = fabsf(r);
   if (r > 1.f) return 0.f;
   return 1.f - r;
 }
-__device__ __forceinline__ float hat3(float rx, float ry, float rz, float s) {
-  return hat_unit(rx / s) * hat_unit(ry / s) * hat_unit(rz / s);
-}
+// Correctly rounded float division by a constant d from inv = 1/d rounded to
+// double: the product x*inv carries a relative error below 2^-52, while the exact
+// quotient of two floats is never a float rounding midpoint and lies more than
+// 2^-49 (relative) away from every midpoint, so rounding the double product to
+// float gives exactly the IEEE quotient x/d for every float x. Three instructions
+// in place of the ~12 of the IEEE division sequence.
+__device__ __forceinline__ float div_exact(float x, double inv) { return (float)((double)x * inv); }
+// trilinearUnitKernel(r / h) with the exact division.
+__device__ __forceinline__ float hat_div(float r, double inv_h) { return hat_unit(div_exact(r, inv_h)); }
 __device__ __forceinline__ float lerpf_(float a, float b, float t) { return a + t * (b - a); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 // Interior cell (x,y,z) of a flat z-fastest index over X*Y*Z.
 __device__ __forceinline__ void cell_of(const Geom& G, int n, int& x, int& y, int& z) {
-  const int yz = G.Y * G.Z;
-  x = n / yz;
-  const int r = n - x * yz;
-  y = r / G.Z;
+  x = (int)fast_div((uint32_t)n, G.divYZ);
+  const int r = n - x * (G.Y * G.Z);
+  y = (int)fast_div((uint32_t)r, G.divZ);
   z = r - y * G.Z;
 }
 // Face cell (x,y,z) of a flat z-fastest index over (X+1)*(Y+1)*(Z+1).
 __device__ __forceinline__ void face_of(const Geom& G, int n, int& x, int& y, int& z) {
-  const int Z1 = G.Z + 1, yz = (G.Y + 1) * Z1;
-  x = n / yz;
-  const int r = n - x * yz;
-  y = r / Z1;
-  z = r - y * Z1;
+  x = (int)fast_div((uint32_t)n, G.divYZ1);
+  const int r = n - x * ((G.Y + 1) * (G.Z + 1));
+  y = (int)fast_div((uint32_t)r, G.divZ1);
+  z = r - y * (G.Z + 1);
 }
 
 // ------------------------------------------------------------------ fill
@@ -76,26 +81,54 @@ void launch_fill_u32(void* p, uint32_t v, long long n, hipStream_t s) {
 // Positions are always inside [dx/4, L - dx/4] after advection and validated on
 // upload, so the clamp to the grid is a no-op there; it only keeps a non-finite
 // position (from a non-finite velocity) from indexing outside the bins.
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 __device__ __forceinline__ int cell_hash(const Geom& G, float px, float py, float pz) {
-  const int x = clampi((int)(px / G.h), 0, G.X - 1);
-  const int y = clampi((int)(py / G.h), 0, G.Y - 1);
-  const int z = clampi((int)(pz / G.h), 0, G.Z - 1);
+  const int x = clampi((int)div_exact(px, G.inv_h), 0, G.X - 1);
+  const int y = clampi((int)div_exact(py, G.inv_h), 0, G.Y - 1);
+  const int z = clampi((int)div_exact(pz, G.inv_h), 0, G.Z - 1);
   return x * (G.Y * G.Z) + y * G.Z + z;
 }
 
-__global__ void k_hash_count(Geom G, const float* __restrict__ px, const float* __restrict__ py,
-                             const float* __restrict__ pz, long long Np, int* __restrict__ hash,
-                             int* __restrict__ count) {
-  const long long i = gtid();
-  if (i >= Np) return;
-  const int hh = cell_hash(G, px[i], py[i], pz[i]);
-  hash[i] = hh;
-  atomicAdd(&count[hh], 1);
+// Bin one particle per lane: lanes of the wave holding the same hash reserve their
+// slots with one atomic (issued by the lowest such lane) and take consecutive
+// ranks in lane order. rank = slot of the particle inside its cell's segment
+// before the segment sort.
+__device__ __forceinline__ void bin_particle(int hh, bool valid, int* __restrict__ count,
+                                             int& rank) {
+  const int lane = (int)(threadIdx.x & 63);
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  unsigned long long pending = __ballot(valid);
+  rank = 0;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lh = __shfl(hh, leader, 64);
+    const unsigned long long same = __ballot(valid && hh == lh) & pending;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[lh], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (valid && hh == lh) rank = base + __popcll(same & lt);
+    pending &= ~same;
+  }
+}
+
+__global__ void k_hash_bin(Geom G, Particles P, long long Np, int* __restrict__ hash,
+                           int* __restrict__ rank, int* __restrict__ count) {
+  const int i = gtid();
+  const bool valid = i < Np;
+  int hh = 0;
+  if (valid) {
+    const float4 a = P.a[i];
+    hh = cell_hash(G, a.x, a.y, a.z);
+  }
+  int r;
+  bin_particle(hh, valid, count, r);
+  if (valid) {
+    hash[i] = hh;
+    rank[i] = r;
+  }
 }
 void launch_hash_count(const Geom& G, Particles P, long long Np, Bins B, hipStream_t s) {
-  hipLaunchKernelGGL(k_hash_count, dim3(nblocks(Np)), dim3(kBlock), 0, s, G, P.c[0], P.c[1],
-                     P.c[2], Np, B.hash, B.count);
+  hipLaunchKernelGGL(k_hash_bin, dim3(nblocks(Np)), dim3(kBlock), 0, s, G, P, Np, B.hash, B.rank,
+                     B.count);
 }
 
 // ------------------------------------------------------------------ scan
@@ -194,23 +227,25 @@ void launch_scan(const Geom& G, Bins B, hipStream_t s) {
 
 // ------------------------------------------------------------------ stable counting sort
 // sort_by_key(hash, index) (Fluid/Fluid_3D_common.cuh:35), stable by previous index.
-// The atomic scatter places each particle in its cell's segment in arbitrary order
-// (and returns every count to zero); each segment is then sorted by previous
-// index, which is the stable order. applySortImpl (Fluid_3D_kernels.cuh:95-102)
-// becomes a gather of the payload channels through the permutation.
-__global__ void k_scatter(const int* __restrict__ hash, long long Np, int* __restrict__ count,
+// Each particle lands at offsets[hash] + rank; ranks are consecutive in index order
+// within a wave, but the waves that share a cell reserve their runs in arbitrary
+// order, so each segment is then sorted by previous index (usually already in
+// order). applySortImpl (Fluid_3D_kernels.cuh:95-102) becomes a gather of the
+// particle records through the permutation.
+__global__ void k_scatter(const int* __restrict__ hash, const int* __restrict__ rank, long long Np,
                           const int* __restrict__ offsets, int* __restrict__ perm) {
-  const long long i = gtid();
+  const int i = gtid();
   if (i >= Np) return;
-  const int hh = hash[i];
-  const int r = atomicSub(&count[hh], 1) - 1;
-  perm[offsets[hh] + r] = (int)i;
+  perm[offsets[hash[i]] + rank[i]] = i;
 }
 
 __global__ void k_segsort(const int* __restrict__ offsets, int ncells, int* __restrict__ perm) {
-  const long long c = gtid();
+  const int c = gtid();
   if (c >= ncells) return;
   const int s = offsets[c], e = offsets[c + 1];
+  bool sorted = true;
+  for (int i = s + 1; i < e; ++i) sorted &= perm[i - 1] < perm[i];
+  if (sorted) return;
   for (int i = s + 1; i < e; ++i) {
     const int v = perm[i];
     int j = i - 1;
@@ -224,73 +259,145 @@ __global__ void k_segsort(const int* __restrict__ offsets, int ncells, int* __re
 
 template <int K>
 __global__ void k_gather(Particles cur, Particles alt, long long Np, const int* __restrict__ perm) {
-  const long long k = gtid();
+  const int k = gtid();
   if (k >= Np) return;
   const int j = perm[k];
-#pragma unroll
-  for (int ch = 0; ch < 6 + K; ++ch) alt.c[ch][k] = cur.c[ch][j];
+  alt.a[k] = cur.a[j];
+  alt.b[k] = cur.b[j];
+  if (K > 2) alt.c[k] = cur.c[j];
 }
 
 void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, long long Np, int K,
                                 Bins B, hipStream_t s) {
-  hipLaunchKernelGGL(k_scatter, dim3(nblocks(Np)), dim3(kBlock), 0, s, B.hash, Np, B.count,
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks(Np)), dim3(kBlock), 0, s, B.hash, B.rank, Np,
                      B.offsets, B.perm);
   hipLaunchKernelGGL(k_segsort, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, B.offsets,
                      G.ncells_hash, B.perm);
-  switch (K) {
-    case 1: hipLaunchKernelGGL(k_gather<1>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm); break;
-    case 2: hipLaunchKernelGGL(k_gather<2>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm); break;
-    case 3: hipLaunchKernelGGL(k_gather<3>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm); break;
-    default: hipLaunchKernelGGL(k_gather<4>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm); break;
-  }
+  if (K > 2)
+    hipLaunchKernelGGL(k_gather<4>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm);
+  else
+    hipLaunchKernelGGL(k_gather<2>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm);
 }
 
 // ------------------------------------------------------------------ P2G
-// One thread per interior cell, fusing resetCellImpl (Fluid_3D_FLIP.cu:287-302),
-// transferToCellAccumPhase (:8-69), transferToCellDividePhase (:72-116) and
-// transferVolumeFractionsToCell (:118-165). Neighbour cells are visited in the
-// reference order (dx, dy, dz from -1 to 1, flattened index range-checked in 1D)
-// and particles in sorted order, so every sum is accumulated in the same order.
+// Fuses resetCellImpl (Fluid_3D_FLIP.cu:287-302), transferToCellAccumPhase (:8-69),
+// transferToCellDividePhase (:72-116) and transferVolumeFractionsToCell (:118-165).
+//
+// One wavefront per block owns a segment of 64 cells along z of one (x, y) row.
+// The reference visits, for each cell, its 27 neighbours in (dx, dy, dz) order by
+// flattened index h = (x+dx)*Y*Z + (y+dy)*Z + z+dz (range-checked in 1D only) and
+// each neighbour's particles in sorted order. For a fixed (dx, dy) the three
+// cells dz = -1, 0, 1 are consecutive hashes, so their particles form one
+// contiguous sorted range per lane, and the ranges of all 64 lanes form one
+// contiguous range per wave (row wrap-around included, exactly as the flattened
+// index wraps). The wave stages that range through LDS with coalesced 16-byte
+// loads; because every lane shares x and y, the x and y hat factors of a staged
+// particle are the same for the whole wave and are multiplied once, as
+// (kx * ky), the first product of trilinearHatKernel. Each lane then forms the
+// z factors and accumulates its range in order, so every sum sees the reference's
+// operands in the reference's order.
+constexpr int kP2GWave = 64;
+constexpr int kP2GChunk = 256;  // particles staged per pass
+
+struct P2GGeom {
+  int nsegz;
+  FastDiv divSeg, divY;
+};
+
 template <int K>
-__global__ void k_p2g(Geom G, Particles P, const int* __restrict__ off, Grid gr, int nv, int hv,
-                      int nf, StepParams sp) {
-  const long long n = gtid();
-  if (n >= G.ncells_hash) return;
-  int x, y, z;
-  cell_of(G, (int)n, x, y, z);
+__global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles P,
+                                                  const int* __restrict__ off, Grid gr, int nv,
+                                                  int hv, int nf, StepParams sp) {
+  __shared__ float4 sA[kP2GChunk];  // (kxF*kyC, kxC*kyF, kxC*kyC, pz)
+  __shared__ float4 sB[kP2GChunk];  // (vx, vy, vz, f0)
+  __shared__ float4 sC[kP2GChunk];  // (f1, f2, f3, -)
+  // block -> (x, y, z segment): b = (x*Y + y)*nsegz + zs
+  const int b = xcd_block();
+  const int xy = (int)fast_div((uint32_t)b, PG.divSeg);
+  const int zs = b - xy * PG.nsegz;
+  const int x = (int)fast_div((uint32_t)xy, PG.divY);
+  const int y = xy - x * G.Y;
+  const int lane = threadIdx.x;
+  const int z0 = zs * kP2GWave;
+  const int z = z0 + lane;
+  const int zl = min(z0 + kP2GWave, G.Z) - 1;  // last cell of the segment
+  const bool active = z < G.Z;
   const float h = G.h;
-  const float* __restrict__ px = P.c[0];
-  const float* __restrict__ py = P.c[1];
-  const float* __restrict__ pz = P.c[2];
-  const float* __restrict__ vx = P.c[3];
-  const float* __restrict__ vy = P.c[4];
-  const float* __restrict__ vz = P.c[5];
+  const double ih = G.inv_h;
   const float fx = (float)x, fy = (float)y, fz = (float)z;
-  const float hx = fx + 0.5f, hy = fy + 0.5f, hz = fz + 0.5f;
-  const float xp0 = fx * h, xp1 = hy * h, xp2 = hz * h;  // u face (x, y+.5, z+.5)
-  const float yp0 = hx * h, yp1 = fy * h, yp2 = hz * h;  // v face (x+.5, y, z+.5)
-  const float zp0 = hx * h, zp1 = hy * h, zp2 = fz * h;  // w face (x+.5, y+.5, z)
-  float tv0 = 0.f, tv1 = 0.f, tv2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
+  const float xF = fx * h, yF = fy * h, zF = fz * h;
+  const float xC = (fx + 0.5f) * h, yC = (fy + 0.5f) * h, zC = (fz + 0.5f) * h;
+  const int NC = G.ncells_hash;
   const int YZ = G.Y * G.Z;
+  const int own = x * YZ + y * G.Z + z;
+  float tv0 = 0.f, tv1 = 0.f, tv2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
+  float f[K];
+#pragma unroll
+  for (int a = 0; a < K; ++a) f[a] = 0.f;
+  float ws = 0.f;
+  int ownLo = 0, ownHi = 0;
+  if (active) {
+    ownLo = off[own];
+    ownHi = off[own + 1];
+  }
   for (int dx = -1; dx <= 1; ++dx)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dz = -1; dz <= 1; ++dz) {
-        const int cell = (x + dx) * YZ + (y + dy) * G.Z + z + dz;
-        if (cell < 0 || cell >= G.ncells_hash) continue;
-        const int s = off[cell], e = off[cell + 1];
-        for (int j = s; j < e; ++j) {
-          const float qx = px[j], qy = py[j], qz = pz[j];
-          const float wx = hat3(qx - xp0, qy - xp1, qz - xp2, h);
-          const float wy = hat3(qx - yp0, qy - yp1, qz - yp2, h);
-          const float wz = hat3(qx - zp0, qy - zp1, qz - zp2, h);
-          tv0 += wx * vx[j];
-          tv1 += wy * vy[j];
-          tv2 += wz * vz[j];
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int h0 = (x + dx) * YZ + (y + dy) * G.Z;  // flattened index of (x+dx, y+dy, 0)
+      const int lo = active ? off[clampi(h0 + z - 1, 0, NC)] : 0;
+      const int hi = active ? off[clampi(h0 + z + 2, 0, NC)] : 0;
+      const int wlo = off[clampi(h0 + z0 - 1, 0, NC)];
+      const int whi = off[clampi(h0 + zl + 2, 0, NC)];
+      const bool centre = (dx == 0) & (dy == 0);
+      for (int cs = wlo; cs < whi; cs += kP2GChunk) {
+        const int ce = min(cs + kP2GChunk, whi);
+        __syncthreads();  // previous chunk fully consumed
+        for (int k = lane; k < ce - cs; k += kP2GWave) {
+          const float4 A = P.a[cs + k];
+          const float4 B = P.b[cs + k];
+          const float kxF = hat_div(A.x - xF, ih), kxC = hat_div(A.x - xC, ih);
+          const float kyF = hat_div(A.y - yF, ih), kyC = hat_div(A.y - yC, ih);
+          sA[k] = make_float4(kxF * kyC, kxC * kyF, kxC * kyC, A.z);
+          sB[k] = make_float4(A.w, B.x, B.y, B.z);
+          if (K > 1) {
+            float4 c = make_float4(B.w, 0.f, 0.f, 0.f);
+            if (K > 2) {
+              const float4 Cc = P.c[cs + k];
+              c.y = Cc.x;
+              c.z = Cc.y;
+            }
+            sC[k] = c;
+          }
+        }
+        __syncthreads();
+        const int j0 = max(lo, cs), j1 = min(hi, ce);
+        for (int j = j0; j < j1; ++j) {
+          const float4 RA = sA[j - cs];
+          const float4 RB = sB[j - cs];
+          const float kzF = hat_div(RA.w - zF, ih), kzC = hat_div(RA.w - zC, ih);
+          const float wx = RA.x * kzC;  // u face (x, y+.5, z+.5)
+          const float wy = RA.y * kzC;  // v face (x+.5, y, z+.5)
+          const float wz = RA.z * kzF;  // w face (x+.5, y+.5, z)
+          tv0 += wx * RB.x;
+          tv1 += wy * RB.y;
+          tv2 += wz * RB.z;
           w0 += wx;
           w1 += wy;
           w2 += wz;
+          if (centre && j >= ownLo && j < ownHi) {
+            const float wc = RA.z * kzC;  // cell centre (x+.5, y+.5, z+.5)
+            f[0] += wc * RB.w;
+            if (K > 1) {
+              const float4 RC = sC[j - cs];
+              f[1] += wc * RC.x;
+              if (K > 2) f[2] += wc * RC.y;
+              if (K > 3) f[3] += wc * RC.z;
+            }
+            ws += wc;
+          }
         }
       }
+    }
+  if (!active) return;
   uint8_t hvb = 0;
   if (w0 > 0.f) { tv0 /= w0; hvb |= 1; }
   if (w1 > 0.f) { tv1 /= w1; hvb |= 2; }
@@ -303,21 +410,7 @@ __global__ void k_p2g(Geom G, Particles P, const int* __restrict__ off, Grid gr,
   gr.nvel[nv][1][gi] = tv1;
   gr.nvel[nv][2][gi] = tv2;
   gr.hasvel[hv][gi] = hvb;
-  const int s0 = off[n], e0 = off[n + 1];
-  gr.content[gi] = (e0 > s0) ? FLUID : AIR;
-
-  // Volume fractions at the cell centre from the cell's own particles.
-  const float cp0 = hx * h, cp1 = hy * h, cp2 = hz * h;
-  float f[K];
-#pragma unroll
-  for (int a = 0; a < K; ++a) f[a] = 0.f;
-  float ws = 0.f;
-  for (int j = s0; j < e0; ++j) {
-    const float w = hat3(px[j] - cp0, py[j] - cp1, pz[j] - cp2, h);
-#pragma unroll
-    for (int a = 0; a < K; ++a) f[a] += w * P.c[6 + a][j];
-    ws += w;
-  }
+  gr.content[gi] = (ownHi > ownLo) ? FLUID : AIR;
   if (ws > 0.f) {
 #pragma unroll
     for (int a = 0; a < K; ++a) f[a] /= ws;
@@ -341,12 +434,16 @@ __global__ void k_p2g(Geom G, Particles P, const int* __restrict__ off, Grid gr,
 void launch_p2g(const Geom& G, Particles P, long long Np, int K, Bins B, Grid gr, int nv, int hv,
                 int nf, StepParams sp, hipStream_t s) {
   (void)Np;
-  const int nb = nblocks(G.ncells_hash);
+  P2GGeom PG;
+  PG.nsegz = (G.Z + kP2GWave - 1) / kP2GWave;
+  PG.divSeg = make_fastdiv((uint32_t)PG.nsegz);
+  PG.divY = make_fastdiv((uint32_t)G.Y);
+  const int nb = G.X * G.Y * PG.nsegz;
   switch (K) {
-    case 1: hipLaunchKernelGGL(k_p2g<1>, dim3(nb), dim3(kBlock), 0, s, G, P, B.offsets, gr, nv, hv, nf, sp); break;
-    case 2: hipLaunchKernelGGL(k_p2g<2>, dim3(nb), dim3(kBlock), 0, s, G, P, B.offsets, gr, nv, hv, nf, sp); break;
-    case 3: hipLaunchKernelGGL(k_p2g<3>, dim3(nb), dim3(kBlock), 0, s, G, P, B.offsets, gr, nv, hv, nf, sp); break;
-    default: hipLaunchKernelGGL(k_p2g<4>, dim3(nb), dim3(kBlock), 0, s, G, P, B.offsets, gr, nv, hv, nf, sp); break;
+    case 1: hipLaunchKernelGGL(k_p2g<1>, dim3(nb), dim3(kP2GWave), 0, s, G, PG, P, B.offsets, gr, nv, hv, nf, sp); break;
+    case 2: hipLaunchKernelGGL(k_p2g<2>, dim3(nb), dim3(kP2GWave), 0, s, G, PG, P, B.offsets, gr, nv, hv, nf, sp); break;
+    case 3: hipLaunchKernelGGL(k_p2g<3>, dim3(nb), dim3(kP2GWave), 0, s, G, PG, P, B.offsets, gr, nv, hv, nf, sp); break;
+    default: hipLaunchKernelGGL(k_p2g<4>, dim3(nb), dim3(kP2GWave), 0, s, G, PG, P, B.offsets, gr, nv, hv, nf, sp); break;
   }
 }
 
@@ -357,11 +454,10 @@ void launch_p2g(const Geom& G, Particles P, long long Np, int K, Bins B, Grid gr
 // since the padding layer is both a gravity target and a boundary face.
 __global__ void k_gravity_boundary(Geom G, Grid gr, int nv, StepParams sp, int doGravity,
                                    int doBoundary) {
-  const long long n = gtid();
-  const long long nf = (long long)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
-  if (n >= nf) return;
+  const int n = gtid();
+  if (n >= G.nfaces) return;
   int x, y, z;
-  face_of(G, (int)n, x, y, z);
+  face_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
   float* __restrict__ U = gr.nvel[nv][0];
   float* __restrict__ V = gr.nvel[nv][1];
@@ -416,9 +512,8 @@ __global__ void k_gravity_boundary(Geom G, Grid gr, int nv, StepParams sp, int d
 }
 void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int doGravity,
                              int doBoundary, hipStream_t s) {
-  const long long nf = (long long)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
-  hipLaunchKernelGGL(k_gravity_boundary, dim3(nblocks(nf)), dim3(kBlock), 0, s, G, gr, nv, sp,
-                     doGravity, doBoundary);
+  hipLaunchKernelGGL(k_gravity_boundary, dim3(nblocks(G.nfaces)), dim3(kBlock), 0, s, G, gr, nv,
+                     sp, doGravity, doBoundary);
 }
 
 // ------------------------------------------------------------------ divergence
@@ -428,10 +523,10 @@ void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int 
 // correction term is evaluated in double as in the reference. Also stores the
 // Jacobi right-hand side -div*density (Fluid_3D_kernels.cu:474-479).
 __global__ void k_divergence(Geom G, Grid gr, int nv, const int* __restrict__ off) {
-  const long long n = gtid();
+  const int n = gtid();
   if (n >= G.ncells_hash) return;
   int x, y, z;
-  cell_of(G, (int)n, x, y, z);
+  cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
   const float* __restrict__ U = gr.nvel[nv][0];
   const float* __restrict__ V = gr.nvel[nv][1];
@@ -442,7 +537,20 @@ __global__ void k_divergence(Geom G, Grid gr, int nv, const int* __restrict__ of
   const double corr = (double)((float)pc - 8.0f) * 0.001;
   div = (float)((double)div - corr);
   gr.div[gi] = div;
-  gr.rhs[gi] = -div * gr.density[gi];
+  gr.rhs[gi] = gr.content[gi] == AIR ? __uint_as_float(kAirRhsBits) : -div * gr.density[gi];
+}
+
+// Right-hand side from divergence, density and content (after a field upload).
+__global__ void k_refresh_rhs(Geom G, Grid gr) {
+  const int n = gtid();
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell_of(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  gr.rhs[gi] = gr.content[gi] == AIR ? __uint_as_float(kAirRhsBits) : -gr.div[gi] * gr.density[gi];
+}
+void launch_refresh_rhs(const Geom& G, Grid gr, hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_rhs, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr);
 }
 void launch_divergence(const Geom& G, Grid gr, int nv, Bins B, hipStream_t s) {
   hipLaunchKernelGGL(k_divergence, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr, nv,
@@ -451,10 +559,10 @@ void launch_divergence(const Geom& G, Grid gr, int nv, Bins B, hipStream_t s) {
 
 // ------------------------------------------------------------------ Jacobi
 __global__ void k_fill_interior_zero(Geom G, float* __restrict__ p) {
-  const long long n = gtid();
+  const int n = gtid();
   if (n >= G.ncells_hash) return;
   int x, y, z;
-  cell_of(G, (int)n, x, y, z);
+  cell_of(G, n, x, y, z);
   p[G.g(x, y, z)] = 0.f;
 }
 void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
@@ -465,14 +573,16 @@ void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
 // jacobiImpl (Fluid/Fluid_3D_kernels.cu:460-505), read-old/write-new: AIR cells 0,
 // otherwise (sum of the six clamp-to-edge neighbours + rhs) / 6, summed in the
 // reference order x+1, x-1, y+1, y-1, z+1, z-1, rhs.
-__global__ void k_jacobi(Geom G, const uint8_t* __restrict__ content, const float* __restrict__ rhs,
-                         const float* __restrict__ pin, float* __restrict__ pout) {
-  const long long n = gtid();
+// AIR cells are recognised from the right-hand side's reserved bit pattern.
+__global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
+                         float* __restrict__ pout) {
+  const int n = gtid();
   if (n >= G.ncells_hash) return;
   int x, y, z;
-  cell_of(G, (int)n, x, y, z);
+  cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
-  if (content[gi] == AIR) {
+  const float b = rhs[gi];
+  if (__float_as_uint(b) == kAirRhsBits) {
     pout[gi] = 0.f;
     return;
   }
@@ -486,23 +596,22 @@ __global__ void k_jacobi(Geom G, const uint8_t* __restrict__ content, const floa
   s += pin[gi - ym];
   s += pin[gi + zp];
   s += pin[gi - zm];
-  s += rhs[gi];
-  s /= 6.f;
-  pout[gi] = s;
+  s += b;
+  pout[gi] = div_exact(s, 1.0 / 6.0);
 }
 void launch_jacobi(const Geom& G, Grid gr, int pin, hipStream_t s) {
-  hipLaunchKernelGGL(k_jacobi, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr.content,
-                     gr.rhs, gr.p[pin], gr.p[pin ^ 1]);
+  hipLaunchKernelGGL(k_jacobi, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr.rhs,
+                     gr.p[pin], gr.p[pin ^ 1]);
 }
 
 // diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
 // Jacobi already leaves AIR cells at 0, so this only matters with 0 pressure sweeps.
 __global__ void k_air_pressure_zero(Geom G, const uint8_t* __restrict__ content,
                                     float* __restrict__ p) {
-  const long long n = gtid();
+  const int n = gtid();
   if (n >= G.ncells_hash) return;
   int x, y, z;
-  cell_of(G, (int)n, x, y, z);
+  cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
   if (content[gi] == AIR) p[gi] = 0.f;
 }
@@ -516,10 +625,10 @@ void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
 // FLUID side get u -= (p - p_other)/rho_used; hasVelocity is overwritten with
 // "face touches fluid".
 __global__ void k_project(Geom G, Grid gr, int nv, int hv, const float* __restrict__ p) {
-  const long long n = gtid();
+  const int n = gtid();
   if (n >= G.ncells_hash) return;
   int x, y, z;
-  cell_of(G, (int)n, x, y, z);
+  cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
   const uint8_t tc = gr.content[gi];
   const float tp = p[gi];
@@ -560,11 +669,10 @@ void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t 
 // passes the reference's epsilon test (epsilon = -1 + 1e-6). Faces outside the
 // launch domain are copied so both buffers stay complete.
 __global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
-  const long long n = gtid();
-  const long long nfc = (long long)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
-  if (n >= nfc) return;
+  const int n = gtid();
+  if (n >= G.nfaces) return;
   int x, y, z;
-  face_of(G, (int)n, x, y, z);
+  face_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
   const float* __restrict__ Ui = gr.nvel[nvin][0];
   const float* __restrict__ Vi = gr.nvel[nvin][1];
@@ -581,8 +689,15 @@ __global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
     return;
   }
   const float eps = (float)(-1 + 1e-6);
-  uint8_t H = Hi[gi];
+  const uint8_t H = Hi[gi];
   float val[3] = {Ui[gi], Vi[gi], Wi[gi]};
+  if (H == 7) {
+    Uo[gi] = val[0];
+    Vo[gi] = val[1];
+    Wo[gi] = val[2];
+    Ho[gi] = H;
+    return;
+  }
   const float* __restrict__ F[3] = {Ui, Vi, Wi};
   // neighbours -x, -y, -z, +x, +y, +z
   const bool ok[6] = {x > 0, y > 0, z > 0, x < G.X - 1, y < G.Y - 1, z < G.Z - 1};
@@ -607,7 +722,8 @@ __global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
       }
     }
     if (cnt > 0) {
-      val[comp] = sum / (float)cnt;
+      const double inv_cnt[7] = {0.0, 1.0, 1.0 / 2.0, 1.0 / 3.0, 1.0 / 4.0, 1.0 / 5.0, 1.0 / 6.0};
+      val[comp] = div_exact(sum, inv_cnt[cnt]);
       Hn |= (uint8_t)(1u << comp);
     }
   }
@@ -617,8 +733,8 @@ __global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
   Ho[gi] = Hn;
 }
 void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s) {
-  const long long nfc = (long long)(G.X + 1) * (G.Y + 1) * (G.Z + 1);
-  hipLaunchKernelGGL(k_extrapolate, dim3(nblocks(nfc)), dim3(kBlock), 0, s, G, gr, nvin, hvin);
+  hipLaunchKernelGGL(k_extrapolate, dim3(nblocks(G.nfaces)), dim3(kBlock), 0, s, G, gr, nvin,
+                     hvin);
 }
 
 // ------------------------------------------------------------------ diffusion
@@ -626,11 +742,11 @@ void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t 
 // f' = (lambda * sum of the six clamp-to-edge neighbours + f) / (1 + 2 lambda),
 // then normalised by its channel sum; AIR cells keep their value.
 template <int K>
-__global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda) {
-  const long long n = gtid();
+__global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda, double inv_cc) {
+  const int n = gtid();
   if (n >= G.ncells_hash) return;
   int x, y, z;
-  cell_of(G, (int)n, x, y, z);
+  cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
   if (gr.content[gi] == AIR) {
 #pragma unroll
@@ -641,7 +757,6 @@ __global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda) {
                            gi + (y < G.Y - 1 ? (long long)G.EZ : 0),
                            gi - (y > 0 ? (long long)G.EZ : 0), gi + (z < G.Z - 1 ? 1 : 0),
                            gi - (z > 0 ? 1 : 0)};
-  const float cc = 1.f + 2.f * lambda;
   float r[K];
 #pragma unroll
   for (int a = 0; a < K; ++a) r[a] = 0.f;
@@ -653,20 +768,21 @@ __global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda) {
 #pragma unroll
   for (int a = 0; a < K; ++a) r[a] += gr.nvf[fin][a][gi];
 #pragma unroll
-  for (int a = 0; a < K; ++a) r[a] /= cc;
+  for (int a = 0; a < K; ++a) r[a] = div_exact(r[a], inv_cc);  // r / (1 + 2 lambda)
   float sum = r[0];
 #pragma unroll
   for (int a = 1; a < K; ++a) sum += r[a];
 #pragma unroll
   for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = r[a] / sum;
 }
-void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, hipStream_t s) {
+void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc,
+                      hipStream_t s) {
   const int nb = nblocks(G.ncells_hash);
   switch (K) {
-    case 1: hipLaunchKernelGGL(k_diffusion<1>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda); break;
-    case 2: hipLaunchKernelGGL(k_diffusion<2>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda); break;
-    case 3: hipLaunchKernelGGL(k_diffusion<3>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda); break;
-    default: hipLaunchKernelGGL(k_diffusion<4>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda); break;
+    case 1: hipLaunchKernelGGL(k_diffusion<1>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc); break;
+    case 2: hipLaunchKernelGGL(k_diffusion<2>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc); break;
+    case 3: hipLaunchKernelGGL(k_diffusion<3>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc); break;
+    default: hipLaunchKernelGGL(k_diffusion<4>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc); break;
   }
 }
 
@@ -694,32 +810,53 @@ __device__ __forceinline__ float interp(const Geom& G, const float* __restrict__
 __device__ __forceinline__ void point_velocity(const Geom& G, const float* U, const float* V,
                                                const float* W, float px, float py, float pz,
                                                float out[3]) {
-  const float x = px / G.h, y = py / G.h, z = pz / G.h;
+  const float x = div_exact(px, G.inv_h), y = div_exact(py, G.inv_h), z = div_exact(pz, G.inv_h);
   out[0] = interp(G, U, x, y - 0.5f, z - 0.5f);
   out[1] = interp(G, V, x - 0.5f, y, z - 0.5f);
   out[2] = interp(G, W, x - 0.5f, y - 0.5f, z);
 }
 
+// One particle record in registers.
+struct PRec {
+  float p[3], v[3], f[4];
+};
+template <int K>
+__device__ __forceinline__ void load_rec(const Particles& P, int i, PRec& r) {
+  const float4 a = P.a[i], b = P.b[i];
+  r.p[0] = a.x, r.p[1] = a.y, r.p[2] = a.z, r.v[0] = a.w;
+  r.v[1] = b.x, r.v[2] = b.y, r.f[0] = b.z, r.f[1] = b.w;
+  r.f[2] = r.f[3] = 0.f;
+  if (K > 2) {
+    const float4 c = P.c[i];
+    r.f[2] = c.x, r.f[3] = c.y;
+  }
+}
+template <int K>
+__device__ __forceinline__ void store_rec(const Particles& P, int i, const PRec& r) {
+  P.a[i] = make_float4(r.p[0], r.p[1], r.p[2], r.v[0]);
+  P.b[i] = make_float4(r.v[1], r.v[2], r.f[0], r.f[1]);
+  if (K > 2) P.c[i] = make_float4(r.f[2], r.f[3], 0.f, 0.f);
+}
+
 // transferToParticlesImpl (Fluid/Fluid_3D_FLIP.cu:189-227). Returns newV at the
 // particle, which is also RK3's first stage.
 template <int K>
-__device__ __forceinline__ void g2p_one(const Geom& G, Particles& P, long long i, const Grid& gr,
-                                        int nv, int nf, float flip, float nvOut[3]) {
-  const float px = P.c[0][i], py = P.c[1][i], pz = P.c[2][i];
+__device__ __forceinline__ void g2p_one(const Geom& G, PRec& r, const Grid& gr, int nv, int nf,
+                                        float flip, float nvOut[3]) {
   float nvp[3], ovp[3];
-  point_velocity(G, gr.nvel[nv][0], gr.nvel[nv][1], gr.nvel[nv][2], px, py, pz, nvp);
-  point_velocity(G, gr.vel[0], gr.vel[1], gr.vel[2], px, py, pz, ovp);
+  point_velocity(G, gr.nvel[nv][0], gr.nvel[nv][1], gr.nvel[nv][2], r.p[0], r.p[1], r.p[2], nvp);
+  point_velocity(G, gr.vel[0], gr.vel[1], gr.vel[2], r.p[0], r.p[1], r.p[2], ovp);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const float change = nvp[c] - ovp[c];
-    const float fl = P.c[3 + c][i] + change;
+    const float fl = r.v[c] + change;
     const float pic = nvp[c];
-    P.c[3 + c][i] = flip * fl + (1.0f - flip) * pic;
+    r.v[c] = flip * fl + (1.0f - flip) * pic;
     nvOut[c] = nvp[c];
   }
-  const int cx = clampi((int)floorf(px / G.h), 0, G.X - 1);
-  const int cy = clampi((int)floorf(py / G.h), 0, G.Y - 1);
-  const int cz = clampi((int)floorf(pz / G.h), 0, G.Z - 1);
+  const int cx = clampi((int)floorf(div_exact(r.p[0], G.inv_h)), 0, G.X - 1);
+  const int cy = clampi((int)floorf(div_exact(r.p[1], G.inv_h)), 0, G.Y - 1);
+  const int cz = clampi((int)floorf(div_exact(r.p[2], G.inv_h)), 0, G.Z - 1);
   const long long gc = G.g(cx, cy, cz);
   float f[K];
 #pragma unroll
@@ -732,14 +869,14 @@ __device__ __forceinline__ void g2p_one(const Geom& G, Particles& P, long long i
     for (int a = 0; a < K; ++a) f[a] /= s;
   }
 #pragma unroll
-  for (int a = 0; a < K; ++a) P.c[6 + a][i] = f[a];
+  for (int a = 0; a < K; ++a) r.f[a] = f[a];
 }
 
 // moveParticlesImpl (Fluid/Fluid_3D_FLIP.cu:229-285): Ralston RK3 on newVelocity,
 // clamp to [dx/4, L - dx/4] with velocity reflection on clamp.
-__device__ __forceinline__ void advect_one(const Geom& G, Particles& P, long long i, const Grid& gr,
-                                           int nv, float dt, const float u1[3], float outPos[3]) {
-  const float b[3] = {P.c[0][i], P.c[1][i], P.c[2][i]};
+__device__ __forceinline__ void advect_one(const Geom& G, PRec& r, const Grid& gr, int nv, float dt,
+                                           const float u1[3]) {
+  const float b[3] = {r.p[0], r.p[1], r.p[2]};
   const float* U = gr.nvel[nv][0];
   const float* V = gr.nvel[nv][1];
   const float* W = gr.nvel[nv][2];
@@ -754,50 +891,67 @@ __device__ __forceinline__ void advect_one(const Geom& G, Particles& P, long lon
   const float L[3] = {(float)G.X * G.h, (float)G.Y * G.h, (float)G.Z * G.h};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    float d = b[c] + dt * ((((u1[c] * 2.f) / 9.f) + ((u2[c] * 3.f) / 9.f)) + ((u3[c] * 4.f) / 9.f));
+    const double ninth = 1.0 / 9.0;
+    float d = b[c] + dt * ((div_exact(u1[c] * 2.f, ninth) + div_exact(u2[c] * 3.f, ninth)) +
+                           div_exact(u3[c] * 4.f, ninth));
     if (d < m) {
       d = m;
-      P.c[3 + c][i] *= -1.f;
+      r.v[c] *= -1.f;
     }
     if (d > L[c] - m) {
       d = L[c] - m;
-      P.c[3 + c][i] *= -1.f;
+      r.v[c] *= -1.f;
     }
-    P.c[c][i] = d;
-    outPos[c] = d;
+    r.p[c] = d;
   }
 }
 
 template <int K>
 __global__ void k_g2p(Geom G, Particles P, long long Np, Grid gr, int nv, int nf, float flip) {
-  const long long i = gtid();
+  const int i = gtid();
   if (i >= Np) return;
+  PRec r;
+  load_rec<K>(P, i, r);
   float u1[3];
-  g2p_one<K>(G, P, i, gr, nv, nf, flip, u1);
+  g2p_one<K>(G, r, gr, nv, nf, flip, u1);
+  store_rec<K>(P, i, r);
 }
 
 __global__ void k_advect(Geom G, Particles P, long long Np, Grid gr, int nv, float dt) {
-  const long long i = gtid();
+  const int i = gtid();
   if (i >= Np) return;
-  float u1[3], pos[3];
-  point_velocity(G, gr.nvel[nv][0], gr.nvel[nv][1], gr.nvel[nv][2], P.c[0][i], P.c[1][i],
-                 P.c[2][i], u1);
-  advect_one(G, P, i, gr, nv, dt, u1, pos);
+  PRec r;
+  load_rec<2>(P, i, r);
+  float u1[3];
+  point_velocity(G, gr.nvel[nv][0], gr.nvel[nv][1], gr.nvel[nv][2], r.p[0], r.p[1], r.p[2], u1);
+  advect_one(G, r, gr, nv, dt, u1);
+  P.a[i] = make_float4(r.p[0], r.p[1], r.p[2], r.v[0]);
+  P.b[i] = make_float4(r.v[1], r.v[2], r.f[0], r.f[1]);
 }
 
-// Fused G2P + RK3 + hash + bin count: one pass over the particles per step.
+// Fused G2P + RK3 + hash + binning: one pass over the particles per step.
 template <int K>
 __global__ void k_g2p_advect_hash(Geom G, Particles P, long long Np, Grid gr, int nv, int nf,
                                   float flip, float dt, int* __restrict__ hash,
-                                  int* __restrict__ count) {
-  const long long i = gtid();
-  if (i >= Np) return;
-  float u1[3], pos[3];
-  g2p_one<K>(G, P, i, gr, nv, nf, flip, u1);
-  advect_one(G, P, i, gr, nv, dt, u1, pos);
-  const int hh = cell_hash(G, pos[0], pos[1], pos[2]);
-  hash[i] = hh;
-  atomicAdd(&count[hh], 1);
+                                  int* __restrict__ rank, int* __restrict__ count) {
+  const int i = gtid();
+  const bool valid = i < Np;
+  int hh = 0;
+  if (valid) {
+    PRec r;
+    load_rec<K>(P, i, r);
+    float u1[3];
+    g2p_one<K>(G, r, gr, nv, nf, flip, u1);
+    advect_one(G, r, gr, nv, dt, u1);
+    store_rec<K>(P, i, r);
+    hh = cell_hash(G, r.p[0], r.p[1], r.p[2]);
+  }
+  int rk;
+  bin_particle(hh, valid, count, rk);
+  if (valid) {
+    hash[i] = hh;
+    rank[i] = rk;
+  }
 }
 
 void launch_g2p(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv, int nf,
@@ -818,10 +972,10 @@ void launch_g2p_advect_hash(const Geom& G, Particles P, long long Np, int K, Gri
                             int nf, StepParams sp, Bins B, hipStream_t s) {
   const int nb = nblocks(Np);
   switch (K) {
-    case 1: hipLaunchKernelGGL(k_g2p_advect_hash<1>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.count); break;
-    case 2: hipLaunchKernelGGL(k_g2p_advect_hash<2>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.count); break;
-    case 3: hipLaunchKernelGGL(k_g2p_advect_hash<3>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.count); break;
-    default: hipLaunchKernelGGL(k_g2p_advect_hash<4>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.count); break;
+    case 1: hipLaunchKernelGGL(k_g2p_advect_hash<1>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count); break;
+    case 2: hipLaunchKernelGGL(k_g2p_advect_hash<2>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count); break;
+    case 3: hipLaunchKernelGGL(k_g2p_advect_hash<3>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count); break;
+    default: hipLaunchKernelGGL(k_g2p_advect_hash<4>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count); break;
   }
 }
 
@@ -848,11 +1002,10 @@ __global__ void k_residual_partial(Geom G, const uint8_t* __restrict__ content,
                                    const float* __restrict__ rhs, const float* __restrict__ p,
                                    double* __restrict__ partial) {
   double rr = 0.0, bb = 0.0;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash;
-       n += stride) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash; n += stride) {
     int x, y, z;
-    cell_of(G, (int)n, x, y, z);
+    cell_of(G, n, x, y, z);
     const long long gi = G.g(x, y, z);
     if (content[gi] != FLUID) continue;
     const double b = (double)rhs[gi];
@@ -909,25 +1062,31 @@ void launch_residual(const Geom& G, Grid gr, int pb, double* partial, int nparts
 // FLUID cells and the particle position bounds (min stored negated so every
 // column folds with a sum or a max).
 constexpr int kStatsWidth = 16;  // f0..f3, fluid cells, -min xyz, max xyz, pad
-__global__ void k_stats_partial(Geom G, Particles P, long long Np, const uint8_t* __restrict__ content,
-                                double* __restrict__ partial) {
+__global__ void k_stats_partial(Geom G, Particles P, long long Np, int K,
+                                const uint8_t* __restrict__ content, double* __restrict__ partial) {
   double f[4] = {0, 0, 0, 0};
   double fluid = 0;
   float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < Np; i += stride) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Np; i += stride) {
+    const float4 a = P.a[i], b = P.b[i];
+    f[0] += (double)b.z;
+    f[1] += (double)b.w;
+    if (K > 2) {
+      const float4 c = P.c[i];
+      f[2] += (double)c.x;
+      f[3] += (double)c.y;
+    }
+    const float q[3] = {a.x, a.y, a.z};
 #pragma unroll
-    for (int a = 0; a < 4; ++a) f[a] += (double)P.c[6 + a][i];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      mn[a] = fminf(mn[a], P.c[a][i]);
-      mx[a] = fmaxf(mx[a], P.c[a][i]);
+    for (int a3 = 0; a3 < 3; ++a3) {
+      mn[a3] = fminf(mn[a3], q[a3]);
+      mx[a3] = fmaxf(mx[a3], q[a3]);
     }
   }
-  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash;
-       n += stride) {
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash; n += stride) {
     int x, y, z;
-    cell_of(G, (int)n, x, y, z);
+    cell_of(G, n, x, y, z);
     if (content[G.g(x, y, z)] == FLUID) fluid += 1.0;
   }
   double col[kStatsWidth];
@@ -965,9 +1124,9 @@ __global__ void k_fold_stats(const double* __restrict__ partial, int nparts, dou
   out[c] = s;
 }
 
-void launch_stats(const Geom& G, Particles P, long long Np, Grid gr, double* partial, int nparts,
-                  double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_stats_partial, dim3(nparts), dim3(kBlock), 0, s, G, P, Np, gr.content,
+void launch_stats(const Geom& G, Particles P, long long Np, int K, Grid gr, double* partial,
+                  int nparts, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_stats_partial, dim3(nparts), dim3(kBlock), 0, s, G, P, Np, K, gr.content,
                      partial);
   hipLaunchKernelGGL(k_fold_stats, dim3(1), dim3(64), 0, s, partial, nparts, out);
 }

@@ -1,0 +1,693 @@
+// unet_kernels.hip -- BF16 MFMA kernels of the pressure warm-start U-Net (gfx950).
+//
+// Tensors are channels-last (B, D, H, W, C) in bf16 with a channel stride and
+// offset, so a decoder's concatenated input is one buffer that the skip
+// connection and the upsampled path write into side by side. All three 3x3x3
+// convolution products -- forward, input gradient and weight gradient -- run on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+//
+// MFMA 16x16x32 lane layout (wave64): lane l supplies row (l % 16) of A and
+// column (l % 16) of B, each with the 8 consecutive k values 8*(l/16) .. +7; it
+// receives C rows 4*(l/16) .. +3 of column (l % 16).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "unet_internal.h"
+
+namespace aqu {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kXCD = 8;
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+__device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }  // round to nearest even
+
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, bid = blockIdx.x;
+  const int per = nb / kXCD, rem = nb % kXCD;
+  const int xcd = bid % kXCD, k = bid / kXCD;
+  return xcd * per + min(xcd, rem) + k;
+}
+
+struct VoxelPos {
+  int b, d, h, w;
+};
+__device__ __forceinline__ VoxelPos vox_of(const Vol& v, long long id) {
+  VoxelPos p;
+  p.w = (int)(id % v.W);
+  long long t = id / v.W;
+  p.h = (int)(t % v.H);
+  t /= v.H;
+  p.d = (int)(t % v.D);
+  p.b = (int)(t / v.D);
+  return p;
+}
+__device__ __forceinline__ long long vox_id(const Vol& v, int b, int d, int h, int w) {
+  return (((long long)b * v.D + d) * v.H + h) * v.W + w;
+}
+
+// ------------------------------------------------------------------ conv 3x3x3, implicit GEMM
+// Y[v, n] = sum_{tap, c} X[v + off(tap), c] * Wk[n, tap*CIN + c] (+ bias, ReLU),
+// zero padding outside the volume. M = voxels (16 per wave), N = COUT, K = 27*CIN
+// in slices of 32. With flipped, transposed weights the same kernel computes the
+// input gradient. OUT_F32 writes FP32 (optionally accumulating), else bf16.
+template <int CIN, int COUT, bool RELU, bool OUT_F32, bool ACCUM>
+__global__ __launch_bounds__(256) void k_conv3(Vol v, const __bf16* __restrict__ X, int xs, int xo,
+                                               const __bf16* __restrict__ Wk,
+                                               const float* __restrict__ bias, void* __restrict__ Yv,
+                                               int ys, int yo) {
+  constexpr int K = 27 * CIN;
+  constexpr int NT = COUT / 16;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long V = (long long)v.B * v.D * v.H * v.W;
+  const long long vbase = ((long long)xcd_block() * 4 + wave) * 16;
+  if (vbase >= V) return;
+  const int r = lane & 15, q = lane >> 4;
+  const long long vid = vbase + r;
+  const bool vok = vid < V;
+  const VoxelPos p = vox_of(v, vok ? vid : 0);
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero8 = {};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const int k = k0 + q * 8;
+    bf16x8 a = zero8;
+    if (k < K && vok) {
+      const int tap = k / CIN, c = k - tap * CIN;
+      const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
+      const int zd = p.d + dd, zh = p.h + dh, zw = p.w + dw;
+      if (zd >= 0 && zd < v.D && zh >= 0 && zh < v.H && zw >= 0 && zw < v.W)
+        a = *(const bf16x8*)(X + vox_id(v, p.b, zd, zh, zw) * xs + xo + c);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      bf16x8 bw = zero8;
+      if (k < K) bw = *(const bf16x8*)(Wk + (long long)(t * 16 + r) * K + k);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int n = t * 16 + r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long long o = vbase + q * 4 + i;
+      if (o >= V) continue;
+      float val = acc[t][i];
+      if (bias) val += bias[n];
+      if (RELU) val = fmaxf(val, 0.f);
+      if (OUT_F32) {
+        float* Y = (float*)Yv + o * ys + yo + n;
+        *Y = ACCUM ? *Y + val : val;
+      } else {
+        ((__bf16*)Yv)[o * ys + yo + n] = f2bf(val);
+      }
+    }
+  }
+}
+
+template <int CIN, int COUT>
+static void conv3_fwd_t(const Vol& v, const __bf16* X, int xs, int xo, const __bf16* Wk,
+                        const float* bias, __bf16* Y, int ys, int yo, bool relu, hipStream_t s) {
+  const long long V = (long long)v.B * v.D * v.H * v.W;
+  const int nb = (int)((V + 63) / 64);
+  if (relu)
+    hipLaunchKernelGGL((k_conv3<CIN, COUT, true, false, false>), dim3(nb), dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
+  else
+    hipLaunchKernelGGL((k_conv3<CIN, COUT, false, false, false>), dim3(nb), dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
+}
+template <int CIN, int COUT>
+static void conv3_dgrad_t(const Vol& v, const __bf16* dY, const __bf16* Wt, float* dX, int dxs, int dxo,
+                          hipStream_t s) {
+  // transposed problem: input dYpre has CIN (= the layer's COUT) channels, output COUT (= its CIN)
+  const long long V = (long long)v.B * v.D * v.H * v.W;
+  const int nb = (int)((V + 63) / 64);
+  hipLaunchKernelGGL((k_conv3<CIN, COUT, false, true, false>), dim3(nb), dim3(256), 0, s, v, dY, CIN, 0, Wt, (const float*)nullptr, (void*)dX, dxs, dxo);
+}
+
+void conv3_forward(int cin, int cout, const Vol& v, const __bf16* X, int xs, int xo, const __bf16* Wk,
+                   const float* bias, __bf16* Y, int ys, int yo, bool relu, hipStream_t s) {
+#define AQU_FWD(CI, CO) \
+  if (cin == CI && cout == CO) return conv3_fwd_t<CI, CO>(v, X, xs, xo, Wk, bias, Y, ys, yo, relu, s);
+  AQU_FWD(8, 16) AQU_FWD(16, 16) AQU_FWD(16, 32) AQU_FWD(32, 32) AQU_FWD(32, 64) AQU_FWD(64, 64)
+  AQU_FWD(96, 32) AQU_FWD(48, 16)
+#undef AQU_FWD
+}
+
+// Input gradient: dX[v, c] = sum_{tap, n} dYpre[v - off(tap), n] * W[n, tap, c], computed as a
+// forward convolution of dYpre (cout channels) with Wt[c, tap', n] = W[n, 26 - tap', c].
+void conv3_dgrad(int cin, int cout, const Vol& v, const __bf16* dY, const __bf16* Wt, float* dX,
+                 int dxs, int dxo, hipStream_t s) {
+#define AQU_DG(CI, CO) \
+  if (cin == CI && cout == CO) return conv3_dgrad_t<CO, CI>(v, dY, Wt, dX, dxs, dxo, s);
+  AQU_DG(16, 16) AQU_DG(16, 32) AQU_DG(32, 32) AQU_DG(32, 64) AQU_DG(64, 64) AQU_DG(96, 32)
+  AQU_DG(48, 16)
+#undef AQU_DG
+}
+
+// ------------------------------------------------------------------ weight gradient
+// dW[n, tap, c] = sum_v dYpre[v, n] * X[v + off(tap), c]: per tap an MFMA GEMM
+// (COUT x CIN) with K = voxels. A block owns one tap and a range of W-rows; for
+// each row it stages dYpre and the tap-shifted X row into LDS transposed
+// (channel-major, voxel contiguous), so a lane's 8 consecutive k (voxels) are one
+// aligned 16-byte LDS read. Partial sums per block go to `part`
+// [split][tap][COUT][CIN] and are folded in a fixed order by k_fold_wgrad.
+constexpr int kWgRow = 64;  // max W handled by the row staging (rows padded to 64)
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_conv3_wgrad(Vol v, const __bf16* __restrict__ dY,
+                                                     const __bf16* __restrict__ X, int xs, int xo,
+                                                     float* __restrict__ part, int nsplit) {
+  constexpr int CINP = CIN < 16 ? 16 : CIN;  // N tiles of 16 (first layer has 8 channels)
+  constexpr int MT = COUT / 16, NTL = CINP / 16, TILES = MT * NTL;
+  constexpr int TPW = (TILES + 3) / 4;  // tiles per wave
+  __shared__ __align__(16) __bf16 sDY[COUT][kWgRow];
+  __shared__ __align__(16) __bf16 sX[CINP][kWgRow];
+  const int split = blockIdx.x, tap = blockIdx.y;
+  const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const long long nrows = (long long)v.B * v.D * v.H;
+  f32x4 acc[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int kch = (v.W + 31) / 32;
+  for (long long row = split; row < nrows; row += nsplit) {
+    const int h = (int)(row % v.H);
+    const long long t = row / v.H;
+    const int d = (int)(t % v.D), b = (int)(t / v.D);
+    const int sd = d + dd, sh = h + dh;
+    const bool rowok = sd >= 0 && sd < v.D && sh >= 0 && sh < v.H;
+    __syncthreads();
+    for (int e = threadIdx.x; e < COUT * kWgRow; e += 256) {
+      const int w = e / COUT, n = e - w * COUT;
+      __bf16 val = f2bf(0.f);
+      if (w < v.W) val = dY[vox_id(v, b, d, h, w) * COUT + n];
+      sDY[n][w] = val;
+    }
+    for (int e = threadIdx.x; e < CINP * kWgRow; e += 256) {
+      const int w = e / CINP, c = e - w * CINP;
+      __bf16 val = f2bf(0.f);
+      const int sw = w + dw;
+      if (rowok && c < CIN && w < v.W && sw >= 0 && sw < v.W)
+        val = X[vox_id(v, b, sd, sh, sw) * xs + xo + c];
+      sX[c][w] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int tile = wave + 4 * i;
+      if (tile >= TILES) break;
+      const int mt = tile / NTL, nt = tile - mt * NTL;
+      for (int kc = 0; kc < kch; ++kc) {
+        const bf16x8 a = *(const bf16x8*)&sDY[mt * 16 + r][kc * 32 + q * 8];
+        const bf16x8 bw = *(const bf16x8*)&sX[nt * 16 + r][kc * 32 + q * 8];
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[i], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    const int tile = wave + 4 * i;
+    if (tile >= TILES) break;
+    const int mt = tile / NTL, nt = tile - mt * NTL;
+    const int c = nt * 16 + r;
+    if (c >= CIN) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = mt * 16 + q * 4 + j;
+      part[(((long long)split * 27 + tap) * COUT + n) * CIN + c] = acc[i][j];
+    }
+  }
+}
+
+// dW[n][tap*CIN + c] = sum over splits (fixed order) of part[split][tap][n][c].
+__global__ void k_fold_wgrad(const float* __restrict__ part, int nsplit, int cin, int cout,
+                             float* __restrict__ dW) {
+  const long long total = 27LL * cout * cin;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int c = (int)(e % cin);
+  const long long t2 = e / cin;
+  const int n = (int)(t2 % cout);
+  const int tap = (int)(t2 / cout);
+  float s = 0.f;
+  for (int sp = 0; sp < nsplit; ++sp) s += part[(((long long)sp * 27 + tap) * cout + n) * cin + c];
+  dW[(long long)n * 27 * cin + tap * cin + c] = s;
+}
+
+template <int CIN, int COUT>
+static void wgrad_t(const Vol& v, const __bf16* dY, const __bf16* X, int xs, int xo, float* part,
+                    int nsplit, float* dW, hipStream_t s) {
+  hipLaunchKernelGGL((k_conv3_wgrad<CIN, COUT>), dim3(nsplit, 27), dim3(256), 0, s, v, dY, X, xs, xo, part, nsplit);
+  const long long total = 27LL * CIN * COUT;
+  hipLaunchKernelGGL(k_fold_wgrad, dim3((int)((total + 255) / 256)), dim3(256), 0, s, part, nsplit, CIN, COUT, dW);
+}
+
+void conv3_wgrad(int cin, int cout, const Vol& v, const __bf16* dY, const __bf16* X, int xs, int xo,
+                 float* part, int nsplit, float* dW, hipStream_t s) {
+#define AQU_WG(CI, CO) \
+  if (cin == CI && cout == CO) return wgrad_t<CI, CO>(v, dY, X, xs, xo, part, nsplit, dW, s);
+  AQU_WG(8, 16) AQU_WG(16, 16) AQU_WG(16, 32) AQU_WG(32, 32) AQU_WG(32, 64) AQU_WG(64, 64)
+  AQU_WG(96, 32) AQU_WG(48, 16)
+#undef AQU_WG
+}
+
+// ------------------------------------------------------------------ ReLU backward + bias gradient
+// dYpre = dY * (Y > 0) as bf16 (dense, C channels), and per-block per-channel sums
+// of dYpre (bias gradient partials, folded by k_fold_rows). Thread t works on
+// channel t % C, so each channel's partial is summed in a fixed order.
+__global__ void k_relu_bwd(long long V, int C, const __bf16* __restrict__ Y, int ysd, int yo,
+                           const float* __restrict__ dY, int dys, int dyo, bool relu,
+                           __bf16* __restrict__ dYpre, float* __restrict__ bpart, int vox_per_block) {
+  __shared__ float red[256];
+  const int c = threadIdx.x % C;
+  const int lanesPerC = 256 / C;
+  const int slot = threadIdx.x / C;
+  float acc = 0.f;
+  const long long v0 = (long long)blockIdx.x * vox_per_block;
+  if (slot < lanesPerC) {
+    for (int i = slot; i < vox_per_block; i += lanesPerC) {
+      const long long vv = v0 + i;
+      if (vv >= V) break;
+      float g = dY[vv * dys + dyo + c];
+      if (relu && !(bf2f(Y[vv * ysd + yo + c]) > 0.f)) g = 0.f;
+      const __bf16 gb = f2bf(g);
+      dYpre[vv * C + c] = gb;
+      acc += bf2f(gb);
+    }
+  }
+  red[threadIdx.x] = (slot < lanesPerC) ? acc : 0.f;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float s = 0.f;
+    for (int k = 0; k < lanesPerC; ++k) s += red[k * C + threadIdx.x];
+    bpart[(long long)blockIdx.x * C + threadIdx.x] = s;
+  }
+}
+
+// out[c] = sum over rows (fixed order) of part[row][c].
+__global__ void k_fold_rows(const float* __restrict__ part, int rows, int C, float* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += part[(long long)r * C + c];
+  out[c] = s;
+}
+
+void relu_bwd(long long V, int C, const __bf16* Y, int ys, int yo, const float* dY, int dys, int dyo,
+              bool relu, __bf16* dYpre, float* bpart, int nblk, float* dbias, hipStream_t s) {
+  const int vpb = (int)((V + nblk - 1) / nblk);
+  hipLaunchKernelGGL(k_relu_bwd, dim3(nblk), dim3(256), 0, s, V, C, Y, ys, yo, dY, dys, dyo, relu,
+                     dYpre, bpart, vpb);
+  hipLaunchKernelGGL(k_fold_rows, dim3((C + 63) / 64), dim3(64), 0, s, bpart, nblk, C, dbias);
+}
+
+// ------------------------------------------------------------------ pooling / upsampling
+// 2x2x2 average pooling (forward, bf16) and its adjoint (accumulating FP32).
+__global__ void k_avgpool(Vol vo, int C, const __bf16* __restrict__ X, int xs, int xo,
+                          __bf16* __restrict__ Y) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long V = (long long)vo.B * vo.D * vo.H * vo.W;
+  if (e >= V * C) return;
+  const int c = (int)(e % C);
+  const VoxelPos p = vox_of(vo, e / C);
+  const Vol vi{vo.B, vo.D * 2, vo.H * 2, vo.W * 2};
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    s += bf2f(X[vox_id(vi, p.b, 2 * p.d + (k >> 2), 2 * p.h + ((k >> 1) & 1), 2 * p.w + (k & 1)) * xs + xo + c]);
+  Y[e] = f2bf(s * 0.125f);
+}
+__global__ void k_avgpool_bwd(Vol vo, int C, const float* __restrict__ dY, float* __restrict__ dX,
+                              int dxs, int dxo) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long V = (long long)vo.B * vo.D * vo.H * vo.W;
+  if (e >= V * C) return;
+  const int c = (int)(e % C);
+  const VoxelPos p = vox_of(vo, e / C);
+  const Vol vi{vo.B, vo.D * 2, vo.H * 2, vo.W * 2};
+  const float g = dY[e] * 0.125f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    float* o = dX + vox_id(vi, p.b, 2 * p.d + (k >> 2), 2 * p.h + ((k >> 1) & 1), 2 * p.w + (k & 1)) * dxs + dxo + c;
+    *o += g;
+  }
+}
+// Nearest 2x upsampling into a channel slice (forward) and its adjoint (sum of the
+// eight children, FP32).
+__global__ void k_upsample(Vol vi, int C, const __bf16* __restrict__ X, __bf16* __restrict__ Y, int ys,
+                           int yo) {
+  const Vol vo{vi.B, vi.D * 2, vi.H * 2, vi.W * 2};
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long V = (long long)vo.B * vo.D * vo.H * vo.W;
+  if (e >= V * C) return;
+  const int c = (int)(e % C);
+  const long long id = e / C;
+  const VoxelPos p = vox_of(vo, id);
+  Y[id * ys + yo + c] = X[vox_id(vi, p.b, p.d >> 1, p.h >> 1, p.w >> 1) * C + c];
+}
+__global__ void k_upsample_bwd(Vol vi, int C, const float* __restrict__ dY, int dys, int dyo,
+                               float* __restrict__ dX) {
+  const Vol vo{vi.B, vi.D * 2, vi.H * 2, vi.W * 2};
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long V = (long long)vi.B * vi.D * vi.H * vi.W;
+  if (e >= V * C) return;
+  const int c = (int)(e % C);
+  const VoxelPos p = vox_of(vi, e / C);
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    s += dY[vox_id(vo, p.b, 2 * p.d + (k >> 2), 2 * p.h + ((k >> 1) & 1), 2 * p.w + (k & 1)) * dys + dyo + c];
+  dX[e] = s;
+}
+
+static inline int nb1(long long n) { return (int)((n + 255) / 256); }
+void avgpool(const Vol& vo, int C, const __bf16* X, int xs, int xo, __bf16* Y, hipStream_t s) {
+  const long long n = (long long)vo.B * vo.D * vo.H * vo.W * C;
+  hipLaunchKernelGGL(k_avgpool, dim3(nb1(n)), dim3(256), 0, s, vo, C, X, xs, xo, Y);
+}
+void avgpool_bwd(const Vol& vo, int C, const float* dY, float* dX, int dxs, int dxo, hipStream_t s) {
+  const long long n = (long long)vo.B * vo.D * vo.H * vo.W * C;
+  hipLaunchKernelGGL(k_avgpool_bwd, dim3(nb1(n)), dim3(256), 0, s, vo, C, dY, dX, dxs, dxo);
+}
+void upsample(const Vol& vi, int C, const __bf16* X, __bf16* Y, int ys, int yo, hipStream_t s) {
+  const long long n = (long long)vi.B * vi.D * vi.H * vi.W * 8 * C;
+  hipLaunchKernelGGL(k_upsample, dim3(nb1(n)), dim3(256), 0, s, vi, C, X, Y, ys, yo);
+}
+void upsample_bwd(const Vol& vi, int C, const float* dY, int dys, int dyo, float* dX, hipStream_t s) {
+  const long long n = (long long)vi.B * vi.D * vi.H * vi.W * C;
+  hipLaunchKernelGGL(k_upsample_bwd, dim3(nb1(n)), dim3(256), 0, s, vi, C, dY, dys, dyo, dX);
+}
+
+// ------------------------------------------------------------------ output head (1x1x1, C -> 1)
+__global__ void k_head(long long V, int C, const __bf16* __restrict__ X, const float* __restrict__ w,
+                       const float* __restrict__ b, float* __restrict__ y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  float s = b[0];
+  for (int c = 0; c < C; ++c) s += w[c] * bf2f(X[i * C + c]);
+  y[i] = s;
+}
+// dX[v, c] = dy[v] * w[c]; per-block partials of dw[c] = sum dy*X and db = sum dy.
+__global__ void k_head_bwd(long long V, int C, const __bf16* __restrict__ X, const float* __restrict__ w,
+                           const float* __restrict__ dy, float* __restrict__ dX, float* __restrict__ part,
+                           int vpb) {
+  __shared__ float red[256 * 17];
+  float acc[17];
+  for (int c = 0; c <= C; ++c) acc[c] = 0.f;
+  const long long v0 = (long long)blockIdx.x * vpb;
+  for (int i = threadIdx.x; i < vpb; i += 256) {
+    const long long vv = v0 + i;
+    if (vv >= V) break;
+    const float g = dy[vv];
+    for (int c = 0; c < C; ++c) {
+      dX[vv * C + c] = g * w[c];
+      acc[c] += g * bf2f(X[vv * C + c]);
+    }
+    acc[C] += g;
+  }
+  for (int c = 0; c <= C; ++c) red[c * 256 + threadIdx.x] = acc[c];
+  __syncthreads();
+  if ((int)threadIdx.x <= C) {
+    float s = 0.f;
+    for (int k = 0; k < 256; ++k) s += red[threadIdx.x * 256 + k];
+    part[(long long)blockIdx.x * (C + 1) + threadIdx.x] = s;
+  }
+}
+void head_fwd(long long V, int C, const __bf16* X, const float* w, const float* b, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_head, dim3(nb1(V)), dim3(256), 0, s, V, C, X, w, b, y);
+}
+void head_bwd(long long V, int C, const __bf16* X, const float* w, const float* dy, float* dX,
+              float* part, int nblk, float* dwb, hipStream_t s) {
+  const int vpb = (int)((V + nblk - 1) / nblk);
+  hipLaunchKernelGGL(k_head_bwd, dim3(nblk), dim3(256), 0, s, V, C, X, w, dy, dX, part, vpb);
+  hipLaunchKernelGGL(k_fold_rows, dim3(1), dim3(64), 0, s, part, nblk, C + 1, dwb);
+}
+
+// ------------------------------------------------------------------ loss
+// Masked mean-squared error over fluid voxels: per-block partials of sum (y-t)^2*m
+// and sum m, folded in order; the gradient dy = 2 (y - t) m / sum m needs the
+// total, so it is a second pass.
+__global__ void k_loss_partial(long long V, const float* __restrict__ y, const float* __restrict__ t,
+                               const float* __restrict__ m, double* __restrict__ part, int vpb) {
+  __shared__ double s1[256], s2[256];
+  double a = 0, c = 0;
+  const long long v0 = (long long)blockIdx.x * vpb;
+  for (int i = threadIdx.x; i < vpb; i += 256) {
+    const long long vv = v0 + i;
+    if (vv >= V) break;
+    const double d = (double)y[vv] - (double)t[vv];
+    a += d * d * m[vv];
+    c += m[vv];
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+__global__ void k_loss_fold(const double* __restrict__ part, int n, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double a = 0, c = 0;
+  for (int i = 0; i < n; ++i) {
+    a += part[2 * i];
+    c += part[2 * i + 1];
+  }
+  out[0] = c > 0 ? a / c : 0.0;
+  out[1] = c;
+}
+__global__ void k_loss_grad(long long V, const float* __restrict__ y, const float* __restrict__ t,
+                            const float* __restrict__ m, const double* __restrict__ tot,
+                            float* __restrict__ dy) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  const double c = tot[1];
+  dy[i] = c > 0 ? (float)(2.0 * ((double)y[i] - (double)t[i]) * m[i] / c) : 0.f;
+}
+void masked_mse(long long V, const float* y, const float* t, const float* m, double* part, int nblk,
+                double* out, float* dy, hipStream_t s) {
+  const int vpb = (int)((V + nblk - 1) / nblk);
+  hipLaunchKernelGGL(k_loss_partial, dim3(nblk), dim3(256), 0, s, V, y, t, m, part, vpb);
+  hipLaunchKernelGGL(k_loss_fold, dim3(1), dim3(64), 0, s, part, nblk, out);
+  if (dy) hipLaunchKernelGGL(k_loss_grad, dim3(nb1(V)), dim3(256), 0, s, V, y, t, m, out, dy);
+}
+
+// ------------------------------------------------------------------ parameters
+// Adam on FP32 master weights (bias-corrected step sizes passed in), then the bf16
+// copies the convolutions read.
+__global__ void k_adam(long long n, float* __restrict__ w, const float* __restrict__ g,
+                       float* __restrict__ m, float* __restrict__ vv, float lr_t, float b1, float b2,
+                       float eps) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * vv[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  vv[i] = vi;
+  w[i] -= lr_t * mi / (sqrtf(vi) + eps);
+}
+void adam(long long n, float* w, const float* g, float* m, float* v, float lr_t, float b1, float b2,
+          float eps, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3(nb1(n)), dim3(256), 0, s, n, w, g, m, v, lr_t, b1, b2, eps);
+}
+
+// Forward weights Wk[n][tap*cin + c] and flipped transposed input-gradient weights
+// Wt[c][tap'*cout + n] = W[n][(26 - tap')*cin + c], both bf16.
+__global__ void k_pack_weights(int cin, int cout, const float* __restrict__ W, __bf16* __restrict__ Wk,
+                               __bf16* __restrict__ Wt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = 27 * cin * cout;
+  if (e >= total) return;
+  const int c = e % cin;
+  const int t2 = e / cin;
+  const int tap = t2 % 27;
+  const int n = t2 / 27;
+  const __bf16 v = f2bf(W[e]);
+  Wk[e] = v;
+  if (Wt) Wt[(long long)c * 27 * cout + (26 - tap) * cout + n] = v;
+}
+void pack_weights(int cin, int cout, const float* W, __bf16* Wk, __bf16* Wt, hipStream_t s) {
+  const int total = 27 * cin * cout;
+  hipLaunchKernelGGL(k_pack_weights, dim3((total + 255) / 256), dim3(256), 0, s, cin, cout, W, Wk, Wt);
+}
+
+// ------------------------------------------------------------------ fills
+__global__ void k_zero_f32(float* __restrict__ p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long st = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = 0.f;
+}
+void zero_f32(float* p, long long n, hipStream_t s) {
+  int nb = (int)std::min<long long>((n + 255) / 256, 8192);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_zero_f32, dim3(nb), dim3(256), 0, s, p, n);
+}
+
+// Probe of the MFMA lane layout assumed above: D = A (16x32) * B (32x16), A and B
+// given row-major in FP32, D row-major.
+__global__ void k_mfma_probe(const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ D) {
+  const int l = threadIdx.x;
+  const int r = l & 15, q = l >> 4;
+  bf16x8 a, b;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    a[j] = f2bf(A[r * 32 + q * 8 + j]);
+    b[j] = f2bf(B[(q * 8 + j) * 16 + r]);
+  }
+  f32x4 c = {0.f, 0.f, 0.f, 0.f};
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) D[(q * 4 + i) * 16 + r] = c[i];
+}
+void mfma_probe(const float* A, const float* B, float* D, hipStream_t s) {
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, A, B, D);
+}
+
+}  // namespace aqu
+
+// ------------------------------------------------------------------ FLIP grid <-> network tensors
+// The network sees the FLIP interior as one volume with (D, H, W) = (X, Y, Z), the
+// same x-major, z-fastest order as the cell hash.
+#include "aqflip_internal.h"
+
+namespace aqu {
+
+__device__ __forceinline__ void cell3(const aqf::Geom& G, long long n, int& x, int& y, int& z) {
+  z = (int)(n % G.Z);
+  const long long t = n / G.Z;
+  y = (int)(t % G.Y);
+  x = (int)(t / G.Y);
+}
+
+// sigma = RMS of the right-hand side b over FLUID cells: sig[0] = sigma, sig[1] = 1/sigma
+// (0 when there is no fluid or b vanishes).
+__global__ void k_nn_sigma_part(aqf::Geom G, const uint8_t* __restrict__ content,
+                                const float* __restrict__ rhs, double* __restrict__ part) {
+  __shared__ double s1[256], s2[256];
+  double a = 0, c = 0;
+  const long long n0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long st = (long long)gridDim.x * blockDim.x;
+  for (long long n = n0; n < G.ncells_hash; n += st) {
+    int x, y, z;
+    cell3(G, n, x, y, z);
+    const long long gi = G.g(x, y, z);
+    if (content[gi] != aqf::FLUID) continue;
+    const double b = (double)rhs[gi];
+    a += b * b;
+    c += 1.0;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+__global__ void k_nn_sigma_fold(const double* __restrict__ part, int n, double* __restrict__ sig) {
+  if (threadIdx.x != 0) return;
+  double a = 0, c = 0;
+  for (int i = 0; i < n; ++i) {
+    a += part[2 * i];
+    c += part[2 * i + 1];
+  }
+  const double s = (c > 0) ? sqrt(a / c) : 0.0;
+  sig[0] = s;
+  sig[1] = s > 0 ? 1.0 / s : 0.0;
+}
+void nn_sigma(const aqf::Geom& G, const uint8_t* content, const float* rhs, double* part, int nparts,
+              double* sig, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_sigma_part, dim3(nparts), dim3(256), 0, s, G, content, rhs, part);
+  hipLaunchKernelGGL(k_nn_sigma_fold, dim3(1), dim3(64), 0, s, part, nparts, sig);
+}
+
+__global__ void k_nn_input(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ rhs,
+                           const double* __restrict__ sig, __bf16* __restrict__ x0, float* __restrict__ mask) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell3(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  const bool fl = content[gi] == aqf::FLUID;
+  const float b = fl ? (float)((double)rhs[gi] * sig[1]) : 0.f;
+  __bf16* o = x0 + n * 8;
+  o[0] = f2bf(b);
+  o[1] = f2bf(fl ? 1.f : 0.f);
+#pragma unroll
+  for (int c = 2; c < 8; ++c) o[c] = f2bf(0.f);
+  if (mask) mask[n] = fl ? 1.f : 0.f;
+}
+void nn_input(const aqf::Geom& G, const uint8_t* content, const float* rhs, const double* sig,
+              __bf16* x0, float* mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_input, dim3((G.ncells_hash + 255) / 256), dim3(256), 0, s, G, content, rhs, sig,
+                     x0, mask);
+}
+
+__global__ void k_nn_target(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ p,
+                            const double* __restrict__ sig, float scale, float* __restrict__ t) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell3(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  const bool fl = content[gi] == aqf::FLUID;
+  t[n] = (fl && sig[0] > 0) ? (float)((double)p[gi] / (sig[0] * (double)scale)) : 0.f;
+}
+void nn_target(const aqf::Geom& G, const uint8_t* content, const float* p, const double* sig, float scale,
+               float* t, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_target, dim3((G.ncells_hash + 255) / 256), dim3(256), 0, s, G, content, p, sig,
+                     scale, t);
+}
+
+__global__ void k_nn_output(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ y,
+                            const double* __restrict__ sig, float scale, float* __restrict__ p) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.ncells_hash) return;
+  int x, yy, z;
+  cell3(G, n, x, yy, z);
+  const long long gi = G.g(x, yy, z);
+  const bool fl = content[gi] == aqf::FLUID;
+  p[gi] = fl ? (float)((double)y[n] * sig[0] * (double)scale) : 0.f;
+}
+void nn_output(const aqf::Geom& G, const uint8_t* content, const float* y, const double* sig, float scale,
+               float* p, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_output, dim3((G.ncells_hash + 255) / 256), dim3(256), 0, s, G, content, y, sig,
+                     scale, p);
+}
+
+// Host tensor (V x 2 floats) -> network input (V x 8 bf16).
+__global__ void k_host_input(long long V, const float* __restrict__ x2, __bf16* __restrict__ x0) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= V) return;
+  __bf16* o = x0 + n * 8;
+  o[0] = f2bf(x2[2 * n]);
+  o[1] = f2bf(x2[2 * n + 1]);
+#pragma unroll
+  for (int c = 2; c < 8; ++c) o[c] = f2bf(0.f);
+}
+void host_input(long long V, const float* x2, __bf16* x0, hipStream_t s) {
+  hipLaunchKernelGGL(k_host_input, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, s, V, x2, x0);
+}
+
+}  // namespace aqu

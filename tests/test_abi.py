@@ -15,13 +15,15 @@ from aquarius_amd import _abi, configs
 from oracle import binding as ob
 
 ROOT = Path(__file__).resolve().parent.parent
-HEADER = ROOT / "include" / "aqflip.h"
+HEADERS = [ROOT / "include" / "aqflip.h", ROOT / "include" / "aqflip_unet.h"]
 
 
 def declared_functions():
-    text = HEADER.read_text()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(aq_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for header in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        names |= set(re.findall(r"\b(aq_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
