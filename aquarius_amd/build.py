@@ -75,10 +75,21 @@ def build_driver() -> Path:
     return DRIVER
 
 
+def build_copy_calibration() -> Path:
+    """Plain 512 MiB device copy used to calibrate HBM traffic counters."""
+    OUT.mkdir(parents=True, exist_ok=True)
+    exe = OUT / "aqflip_copy_calibration"
+    tmp = exe.with_suffix(".tmp")
+    _run([HIPCC, *HIP_FLAGS, str(CSRC / "host" / "copy_calibration.hip"), "-o", str(tmp)])
+    os.replace(tmp, exe)
+    return exe
+
+
 def build(lib_only: bool = False) -> None:
     build_library()
     if not lib_only:
         build_driver()
+        build_copy_calibration()
 
 
 if __name__ == "__main__":

@@ -83,6 +83,7 @@ struct Grid {
   float* density;
   float* vf[4];         // volumeFractions
   float* nvf[2][4];     // newVolumeFractions, ping-pong
+  uint8_t* active;      // per 256-cell launch block of the interior: any FLUID cell
 };
 
 struct Bins {
@@ -116,11 +117,15 @@ void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int 
 void launch_divergence(const Geom& G, Grid gr, int nv, Bins B, hipStream_t s);
 void launch_refresh_rhs(const Geom& G, Grid gr, hipStream_t s);
 void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
-void launch_jacobi(const Geom& G, Grid gr, int pin, hipStream_t s);
+// full = 1: every cell written (AIR cells get 0); full = 0: blocks without FLUID
+// cells and AIR cells are skipped -- valid once both buffers hold 0 on AIR.
+void launch_jacobi(const Geom& G, Grid gr, int pin, int full, hipStream_t s);
 void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s);
 void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s);
-void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc,
+// full = 1: AIR cells copied to the output buffer; full = 0: AIR cells and blocks
+// without FLUID skipped -- valid once both buffers agree on AIR cells.
+void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc, int full,
                       hipStream_t s);
 void launch_g2p(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv, int nf,
                 StepParams sp, hipStream_t s);

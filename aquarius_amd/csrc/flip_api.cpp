@@ -73,6 +73,8 @@ struct aq_flip {
   int cur = 0;
   Grid gr{};
   int nv = 0, hv = 0, pb = 0, nf = 0;  // current buffer of each ping-pong pair
+  // air_clean[i]: pressure buffer i holds 0 on every AIR cell of the current content.
+  bool air_clean[2] = {false, false};
   Bins B{};
   StepParams sp{};
   int nparts_red = 0;
@@ -97,8 +99,8 @@ struct aq_flip {
   // Captured steps: one executable graph per ping-pong state at step entry (odd
   // pass counts alternate buffers between steps), with the state it leaves.
   struct StepGraph {
-    int key[5];
-    int next[5];
+    int key[7];
+    int next[7];
     hipGraph_t graph;
     hipGraphExec_t exec;
   };
@@ -272,6 +274,7 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   AQ_TRY(h->allocT(&gr.div, nc));
   AQ_TRY(h->allocT(&gr.rhs, nc));
   AQ_TRY(h->allocT(&gr.density, nc));
+  AQ_TRY(h->allocT(&gr.active, (size_t)(G.ncells_hash + 255) / 256 + 1));
   for (int a = 0; a < 4; ++a) {
     AQ_TRY(h->allocT(&gr.vf[a], nc));
     AQ_TRY(h->allocT(&gr.nvf[0][a], nc));
@@ -314,12 +317,16 @@ aq_status upload_particles(aq_flip* h, const float* pos, const float* vel, const
 
 // ---- stages --------------------------------------------------------------
 
+// New FLUID/AIR marking: AIR cells of either pressure buffer may hold old values.
+void content_changed(aq_flip* h) { h->air_clean[0] = h->air_clean[1] = false; }
+
 // hash is already binned (count filled): scan, scatter, segment sort, gather, P2G.
 void transfer_after_count(aq_flip* h) {
   launch_scan(h->G, h->B, h->stream);
   launch_scatter_sort_gather(h->G, h->P[h->cur], h->P[h->cur ^ 1], h->Np, h->K, h->B, h->stream);
   h->cur ^= 1;
   launch_p2g(h->G, h->P[h->cur], h->Np, h->K, h->B, h->gr, h->nv, h->hv, h->nf, h->sp, h->stream);
+  content_changed(h);
 }
 
 void clear_bins(aq_flip* h) { launch_fill_u32(h->B.count, 0u, h->G.ncells_hash, h->stream); }
@@ -330,15 +337,27 @@ void stage_transfer(aq_flip* h) {
   transfer_after_count(h);
 }
 
+// n Jacobi sweeps. Every sweep leaves 0 on all AIR cells of its output buffer, so
+// a sweep whose output buffer already holds those zeros may skip AIR cells and
+// blocks without FLUID (the result is identical); otherwise it writes every cell.
 void jacobi_sweeps(aq_flip* h, int n) {
   for (int i = 0; i < n; ++i) {
-    launch_jacobi(h->G, h->gr, h->pb, h->stream);
-    h->pb ^= 1;
+    const int out = h->pb ^ 1;
+    launch_jacobi(h->G, h->gr, h->pb, h->air_clean[out] ? 0 : 1, h->stream);
+    h->air_clean[out] = true;
+    h->pb = out;
   }
 }
 
+// Zero pressure start: both buffers cleared.
+void zero_pressure(aq_flip* h) {
+  launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
+  launch_fill_pressure_zero(h->G, h->gr, h->pb ^ 1, h->stream);
+  h->air_clean[0] = h->air_clean[1] = true;
+}
+
 void stage_pressure(aq_flip* h) {
-  if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
+  if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) zero_pressure(h);
   jacobi_sweeps(h, h->cfg.pressure_iters);
 }
 
@@ -358,8 +377,8 @@ void stage_diffusion(aq_flip* h, bool inStep) {
   if (iters <= 0) return;
   if (!inStep || h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
   if (inStep && h->K == 1) return;
-  for (int i = 0; i < iters; ++i) {
-    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, h->stream);
+  for (int i = 0; i < iters; ++i) {  // the first sweep also copies AIR cells across
+    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, h->stream);
     h->nf ^= 1;
   }
 }
@@ -435,7 +454,7 @@ aq_status sweep_to_tol(aq_flip* h, int budget, bool detect_stall, int* sweeps, d
 aq_status pressure_stage(aq_flip* h) {
   aq_flip_solver_stats& st = h->sstats;
   if (h->solver.mode == AQ_SOLVE_FIXED) {
-    if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
+    if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) zero_pressure(h);
     h->begin(AQ_STAGE_PRESSURE);  // the Jacobi sweeps only
     jacobi_sweeps(h, h->cfg.pressure_iters);
     h->end(AQ_STAGE_PRESSURE);
@@ -448,9 +467,12 @@ aq_status pressure_stage(aq_flip* h) {
   if (h->solver.mode == AQ_SOLVE_TOL_NN) {
     if (!h->net) return fail(AQ_EINVAL, "AQ_SOLVE_TOL_NN needs a U-Net");
     AQ_TRY(aq_flip_nn_guess(h, h->net));
+    h->air_clean[h->pb] = true;  // the guess is 0 on AIR cells
     AQ_TRY(residual_now(h, &guess));
-    // A guess that is non-finite or no better than zero (relative residual 1) is rejected.
-    if (!std::isfinite(guess) || guess >= 1.0) fallback = true;
+    // A non-finite guess is rejected outright. A finite one is kept even when its
+    // residual exceeds zero's (1): its error is mostly high-frequency, which the first
+    // sweeps remove; a guess that then stalls falls back below.
+    if (!std::isfinite(guess)) fallback = true;
     if (!fallback) {
       AQ_TRY(sweep_to_tol(h, h->solver.max_iters, true, &sweeps, &res, &stalled));
       total += sweeps;
@@ -458,7 +480,7 @@ aq_status pressure_stage(aq_flip* h) {
     }
   }
   if (h->solver.mode == AQ_SOLVE_TOL_ZERO || fallback) {
-    launch_fill_pressure_zero(h->G, h->gr, h->pb, h->stream);
+    zero_pressure(h);
     AQ_TRY(sweep_to_tol(h, std::max(0, h->solver.max_iters), false, &sweeps, &res, &stalled));
     total += sweeps;
     if (h->solver.mode == AQ_SOLVE_TOL_ZERO) guess = 1.0;
@@ -515,20 +537,25 @@ void drop_graphs(aq_flip* h) {
   h->graphs.clear();
 }
 
-void state_of(const aq_flip* h, int k[5]) {
+void state_of(const aq_flip* h, int k[7]) {
   k[0] = h->cur, k[1] = h->nv, k[2] = h->hv, k[3] = h->pb, k[4] = h->nf;
+  k[5] = h->air_clean[0], k[6] = h->air_clean[1];
+}
+void set_state(aq_flip* h, const int k[7]) {
+  h->cur = k[0], h->nv = k[1], h->hv = k[2], h->pb = k[3], h->nf = k[4];
+  h->air_clean[0] = k[5] != 0, h->air_clean[1] = k[6] != 0;
 }
 
 // One step through a captured graph (eager launches when timing is on, so the
 // stage events sit on the stream between kernels).
 aq_status graph_step(aq_flip* h) {
   if (h->timing || !h->use_graphs || h->solver.mode != AQ_SOLVE_FIXED) return enqueue_step(h);
-  int key[5];
+  int key[7];
   state_of(h, key);
   for (auto& g : h->graphs) {
     if (std::memcmp(g.key, key, sizeof(key)) == 0) {
       HIP_TRY(hipGraphLaunch(g.exec, h->stream));
-      h->cur = g.next[0], h->nv = g.next[1], h->hv = g.next[2], h->pb = g.next[3], h->nf = g.next[4];
+      set_state(h, g.next);
       h->physicalTime += h->cfg.dt;
       return AQ_OK;
     }
@@ -899,6 +926,7 @@ aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t b
       AQ_TRY(d2h(h, c, h->gr.content, nc));
       each([&](size_t o, size_t g) { c[g] = (uint8_t)I[o]; });
       AQ_TRY(h2d(h, h->gr.content, c));
+      content_changed(h);
       launch_refresh_rhs(h->G, h->gr, h->stream);
       HIP_TRY(hipStreamSynchronize(h->stream));
       break;
@@ -924,6 +952,7 @@ aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t b
       AQ_TRY(d2h(h, v, d, nc));
       each([&](size_t o, size_t g) { v[g] = F[o]; });
       AQ_TRY(h2d(h, d, v));
+      if (field == AQ_FIELD_PRESSURE) h->air_clean[h->pb] = false;
       if (field != AQ_FIELD_PRESSURE) {
         // keep the Jacobi right-hand side consistent with divergence*density
         launch_refresh_rhs(h->G, h->gr, h->stream);

@@ -522,9 +522,20 @@ void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int 
 // computeDivergenceImpl (Fluid_3D_kernels.cu:426-458) with restPPC = 8, whose
 // correction term is evaluated in double as in the reference. Also stores the
 // Jacobi right-hand side -div*density (Fluid_3D_kernels.cu:474-479).
+// Marks the launch block of the calling thread active if any of its cells is FLUID.
+// Every thread of the block must reach it (no early return before).
+__device__ __forceinline__ void mark_active(const Grid& gr, bool fluid) {
+  const int any = __syncthreads_or(fluid ? 1 : 0);
+  if (threadIdx.x == 0) gr.active[xcd_block()] = any ? 1 : 0;
+}
+
 __global__ void k_divergence(Geom G, Grid gr, int nv, const int* __restrict__ off) {
   const int n = gtid();
-  if (n >= G.ncells_hash) return;
+  const bool in = n < G.ncells_hash;
+  int fx = 0, fy = 0, fz = 0;
+  if (in) cell_of(G, n, fx, fy, fz);
+  mark_active(gr, in && gr.content[G.g(fx, fy, fz)] == FLUID);
+  if (!in) return;
   int x, y, z;
   cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
@@ -537,13 +548,18 @@ __global__ void k_divergence(Geom G, Grid gr, int nv, const int* __restrict__ of
   const double corr = (double)((float)pc - 8.0f) * 0.001;
   div = (float)((double)div - corr);
   gr.div[gi] = div;
-  gr.rhs[gi] = gr.content[gi] == AIR ? __uint_as_float(kAirRhsBits) : -div * gr.density[gi];
+  const bool air = gr.content[gi] == AIR;
+  gr.rhs[gi] = air ? __uint_as_float(kAirRhsBits) : -div * gr.density[gi];
 }
 
 // Right-hand side from divergence, density and content (after a field upload).
 __global__ void k_refresh_rhs(Geom G, Grid gr) {
   const int n = gtid();
-  if (n >= G.ncells_hash) return;
+  const bool in = n < G.ncells_hash;
+  int fx = 0, fy = 0, fz = 0;
+  if (in) cell_of(G, n, fx, fy, fz);
+  mark_active(gr, in && gr.content[G.g(fx, fy, fz)] == FLUID);
+  if (!in) return;
   int x, y, z;
   cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
@@ -575,7 +591,8 @@ void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
 // reference order x+1, x-1, y+1, y-1, z+1, z-1, rhs.
 // AIR cells are recognised from the right-hand side's reserved bit pattern.
 __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
-                         float* __restrict__ pout) {
+                         float* __restrict__ pout, const uint8_t* __restrict__ active, int full) {
+  if (!full && !active[xcd_block()]) return;
   const int n = gtid();
   if (n >= G.ncells_hash) return;
   int x, y, z;
@@ -583,7 +600,7 @@ __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __r
   const long long gi = G.g(x, y, z);
   const float b = rhs[gi];
   if (__float_as_uint(b) == kAirRhsBits) {
-    pout[gi] = 0.f;
+    if (full) pout[gi] = 0.f;
     return;
   }
   const long long xp = x < G.X - 1 ? G.sx : 0, xm = x > 0 ? G.sx : 0;
@@ -599,9 +616,9 @@ __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __r
   s += b;
   pout[gi] = div_exact(s, 1.0 / 6.0);
 }
-void launch_jacobi(const Geom& G, Grid gr, int pin, hipStream_t s) {
+void launch_jacobi(const Geom& G, Grid gr, int pin, int full, hipStream_t s) {
   hipLaunchKernelGGL(k_jacobi, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr.rhs,
-                     gr.p[pin], gr.p[pin ^ 1]);
+                     gr.p[pin], gr.p[pin ^ 1], gr.active, full);
 }
 
 // diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
@@ -742,15 +759,18 @@ void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t 
 // f' = (lambda * sum of the six clamp-to-edge neighbours + f) / (1 + 2 lambda),
 // then normalised by its channel sum; AIR cells keep their value.
 template <int K>
-__global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda, double inv_cc) {
+__global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda, double inv_cc, int full) {
+  if (!full && !gr.active[xcd_block()]) return;
   const int n = gtid();
   if (n >= G.ncells_hash) return;
   int x, y, z;
   cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
   if (gr.content[gi] == AIR) {
+    if (full) {
 #pragma unroll
-    for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = gr.nvf[fin][a][gi];
+      for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = gr.nvf[fin][a][gi];
+    }
     return;
   }
   const long long nb[6] = {gi + (x < G.X - 1 ? G.sx : 0), gi - (x > 0 ? G.sx : 0),
@@ -775,14 +795,14 @@ __global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda, double inv_c
 #pragma unroll
   for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = r[a] / sum;
 }
-void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc,
+void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc, int full,
                       hipStream_t s) {
   const int nb = nblocks(G.ncells_hash);
   switch (K) {
-    case 1: hipLaunchKernelGGL(k_diffusion<1>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc); break;
-    case 2: hipLaunchKernelGGL(k_diffusion<2>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc); break;
-    case 3: hipLaunchKernelGGL(k_diffusion<3>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc); break;
-    default: hipLaunchKernelGGL(k_diffusion<4>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc); break;
+    case 1: hipLaunchKernelGGL(k_diffusion<1>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, full); break;
+    case 2: hipLaunchKernelGGL(k_diffusion<2>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, full); break;
+    case 3: hipLaunchKernelGGL(k_diffusion<3>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, full); break;
+    default: hipLaunchKernelGGL(k_diffusion<4>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, full); break;
   }
 }
 

@@ -1,0 +1,164 @@
+"""Learned pressure warm start: training data, training and evaluation.
+
+Training pairs come from the plain solver: randomized ball-drop scenes (1-3
+spheres of radius U[0.1, 0.25] above a pool of depth U[0.1, 0.4], seeded) are
+stepped with the fixed-sweep solver, and at chosen steps the step's pressure
+system is re-solved from zero with plain FP32 Jacobi to a tight FP64 residual;
+(right-hand side, FLUID mask, converged pressure) is captured on the device.
+
+    python -m aquarius_amd.warmstart train --pairs 4000 --out aquarius_amd/weights/unet_warmstart_64.npz
+    python -m aquarius_amd.warmstart compare --weights aquarius_amd/weights/unet_warmstart_64.npz --n 128
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import configs
+from ._abi import AQ_SHAPE_BOX, AQ_SHAPE_SPHERE
+from .fluid import FlipHandle
+from .unet import UNet
+
+WEIGHTS = Path(__file__).resolve().parent / "weights" / "unet_warmstart_64.npz"
+
+
+def random_scene(rng: np.random.Generator, n: int = 64, seed: int = 0):
+    """Pool of depth U[0.1, 0.4] plus 1-3 spheres of radius U[0.1, 0.25] above it."""
+    depth = float(rng.uniform(0.1, 0.4))
+    k = int(rng.integers(1, 4))
+    cfg = configs.make_config(n, dt=0.005, pressure_iters=100, diffusion_iters=0, pool_depth=depth,
+                              seed=seed)
+    vols = [dict(shape=AQ_SHAPE_BOX, box_min=(0, 0, 0), box_max=(1, depth, 1), phase=0)]
+    for _ in range(k):
+        r = float(rng.uniform(0.1, 0.25))
+        lo_y = min(depth + r, 1 - r)
+        c = (float(rng.uniform(r, 1 - r)), float(rng.uniform(lo_y, 1 - r)), float(rng.uniform(r, 1 - r)))
+        vols.append(dict(shape=AQ_SHAPE_SPHERE, center=c, radius=r, phase=0))
+    configs._set_volumes(cfg, vols)
+    return cfg
+
+
+def converge(h: FlipHandle, tol: float, cap: int, check_every: int = 100):
+    """Re-solve the handle's current pressure system from zero with plain Jacobi."""
+    h.set_field("pressure", np.zeros(h.field_shape("pressure"), np.float32))
+    return h.solve_pressure(tol, cap, check_every)
+
+
+def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int = 4,
+             max_step: int = 60, tol: float = 1e-5, accept: float = 1e-4, cap: int = 60000,
+             log=print):
+    """Fill the network's device dataset with `pairs` converged systems."""
+    net.reserve(pairs, (n, n, n))
+    rng = np.random.default_rng(seed)
+    scene = 0
+    t0 = time.time()
+    sweeps = 0
+    while net.dataset_size < pairs:
+        cfg = random_scene(rng, n, seed=int(rng.integers(2**31)))
+        h = FlipHandle(cfg)
+        stops = sorted(set(int(s) for s in rng.integers(1, max_step + 1, per_scene)))
+        done = 0
+        for s in stops:
+            h.steps(s - done)
+            done = s
+            it, rel, _ = converge(h, tol, cap)
+            sweeps += it
+            if rel <= accept and net.dataset_size < pairs:
+                net.add_sample(h)
+        h.close()
+        scene += 1
+        if scene % 50 == 0:
+            log(f"  scenes {scene}, pairs {net.dataset_size}/{pairs}, mean sweeps "
+                f"{sweeps / max(1, net.dataset_size):.0f}, {time.time() - t0:.0f} s")
+    return dict(scenes=scene, pairs=net.dataset_size, seconds=time.time() - t0,
+                mean_sweeps=sweeps / max(1, net.dataset_size))
+
+
+def compare_on(h: FlipHandle, net: UNet, tol: float = 1e-4, cap: int = 200000, check_every: int = 10):
+    """Sweeps to `tol` for the handle's current system: from zero and from the U-Net guess."""
+    from .unet import nn_guess
+    it0, r0, ok0 = converge(h, tol, cap, check_every)
+    nn_guess(h, net)
+    g = h.residual()
+    it1, r1, ok1 = h.solve_pressure(tol, cap, check_every)
+    return dict(plain_iters=it0, plain_residual=r0, plain_ok=ok0, nn_guess_residual=g,
+                nn_iters=it1, nn_residual=r1, nn_ok=ok1)
+
+
+def evaluate(net: UNet, n: int = 64, scenes: int = 8, seed: int = 12345, steps: int = 30,
+             tol: float = 1e-4, log=print):
+    rng = np.random.default_rng(seed)
+    rows = []
+    for _ in range(scenes):
+        cfg = random_scene(rng, n, seed=int(rng.integers(2**31)))
+        h = FlipHandle(cfg)
+        h.steps(int(rng.integers(1, steps + 1)))
+        rows.append(compare_on(h, net, tol))
+        h.close()
+        log("  ", rows[-1])
+    p = np.array([r["plain_iters"] for r in rows], float)
+    q = np.array([r["nn_iters"] for r in rows], float)
+    return dict(rows=rows, plain_mean=float(p.mean()), nn_mean=float(q.mean()),
+                ratio=float(p.sum() / max(q.sum(), 1)))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--pairs", type=int, default=4000)
+    t.add_argument("--held-out", type=int, default=64)
+    t.add_argument("--n", type=int, default=64)
+    t.add_argument("--steps", type=int, default=3000)
+    t.add_argument("--batch", type=int, default=8)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--out", type=Path, default=WEIGHTS)
+    t.add_argument("--eval-scenes", type=int, default=8)
+    c = sub.add_parser("compare")
+    c.add_argument("--weights", type=Path, default=WEIGHTS)
+    c.add_argument("--n", type=int, default=64)
+    c.add_argument("--scenes", type=int, default=4)
+    a = ap.parse_args(argv)
+
+    if a.cmd == "train":
+        net = UNet(lr=a.lr, seed=a.seed + 1)
+        total = a.pairs + a.held_out
+        print(f"[warmstart] generating {total} pairs at {a.n}^3", flush=True)
+        gen = generate(net, total, n=a.n, seed=a.seed)
+        print(f"[warmstart] data: {json.dumps(gen)}", flush=True)
+        train_n = a.pairs
+        l0 = net.eval(train_n, a.held_out)
+        t0 = time.time()
+        hist = []
+        chunk = 250
+        for s in range(0, a.steps, chunk):
+            k = min(chunk, a.steps - s)
+            h = net.train(k, a.batch, train_n, seed=a.seed * 7919 + s)
+            hist.append(float(h.mean()))
+            print(f"[warmstart] step {s + k}: train loss {hist[-1]:.4e} "
+                  f"held-out {net.eval(train_n, a.held_out):.4e} ({time.time() - t0:.0f} s)", flush=True)
+        l1 = net.eval(train_n, a.held_out)
+        a.out.parent.mkdir(parents=True, exist_ok=True)
+        net.save(a.out)
+        print(f"[warmstart] held-out loss {l0:.4e} -> {l1:.4e}; saved {a.out}", flush=True)
+        ev = evaluate(net, n=a.n, scenes=a.eval_scenes)
+        print(f"[warmstart] sweeps to 1e-4: plain {ev['plain_mean']:.0f}, warm {ev['nn_mean']:.0f}, "
+              f"ratio {ev['ratio']:.2f}", flush=True)
+        return 0
+    if a.cmd == "compare":
+        net = UNet()
+        net.load_file(a.weights)
+        ev = evaluate(net, n=a.n, scenes=a.scenes)
+        print(json.dumps({k: v for k, v in ev.items() if k != "rows"}), flush=True)
+        return 0
+    return 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

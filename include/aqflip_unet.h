@@ -15,8 +15,8 @@
  *     is the Jacobi right-hand side and sigma its RMS over FLUID cells;
  *   - pressure-solve modes for aq_flip_step: fixed sweep count (reference), to a
  *     tolerance from zero, or to a tolerance from the U-Net guess with plain
- *     zero-guess Jacobi as the fallback when the warm start is non-finite, worse
- *     than zero or stalls.
+ *     zero-guess Jacobi as the fallback when the warm start is non-finite or its
+ *     refinement stalls.
  * Conventions as in aqflip.h (status codes, thread-local aq_last_error()).
  */
 #ifndef AQFLIP_UNET_H

@@ -1,0 +1,180 @@
+"""The warm-start U-Net: its CPU oracle pinned by finite differences and closed
+forms (CPU), and the GPU forward/backward/Adam kernels checked against the oracle
+(GPU). The GPU keeps bf16 activations and weights with FP32 accumulation; the
+oracle rounds to bf16 at the same points, so the bar is a relative L2 error of
+1e-2 on outputs and 3e-2 on gradients (bf16 has an 8-bit significand)."""
+import numpy as np
+import pytest
+
+from oracle import unet_ref as U
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def random_params(seed=0):
+    rng = np.random.default_rng(seed)
+    blocks, hw, hb, total = U.layout()
+    p = np.zeros(total)
+    for (w, b), (ci, co) in zip(blocks, U.CONVS):
+        real = 2 if ci == 8 else ci
+        W = rng.standard_normal((co, 27, ci)) * np.sqrt(2.0 / (27 * real))
+        if ci == 8:
+            W[:, :, 2:] = 0
+        p[w:w + W.size] = W.ravel()
+        p[b:b + co] = 0.05 * rng.standard_normal(co)
+    p[hw:hw + 16] = rng.standard_normal(16) * 0.25
+    p[hb] = 0.1
+    return p
+
+
+def random_problem(B=2, D=8, H=8, W=8, seed=1):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((B, D, H, W, 2))
+    m = (rng.random((B, D, H, W)) < 0.6).astype(np.float64)
+    x[..., 0] = rng.standard_normal((B, D, H, W)) * m
+    x[..., 1] = m
+    t = rng.standard_normal((B, D, H, W)) * m
+    return x, t, m
+
+
+# ---------------------------------------------------------------- CPU: the oracle itself
+def test_conv_matches_direct_sum():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((1, 4, 4, 4, 3))
+    W = rng.standard_normal((2, 27, 3))
+    b = rng.standard_normal(2)
+    y = U.conv3(x, W, b)
+    # brute force at a corner and an interior voxel
+    for (d, h, w) in [(0, 0, 0), (2, 1, 3)]:
+        want = b.copy()
+        for t, (dd, dh, dw) in enumerate(U.TAPS):
+            z = (d + dd, h + dh, w + dw)
+            if all(0 <= z[i] < 4 for i in range(3)):
+                want += W[:, t, :] @ x[0, z[0], z[1], z[2]]
+        np.testing.assert_allclose(y[0, d, h, w], want, rtol=1e-12)
+
+
+def test_adjoints_of_conv_pool_upsample():
+    """<A x, y> = <x, A^T y> for the convolution, pooling and upsampling pairs."""
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((1, 4, 4, 4, 3))
+    W = rng.standard_normal((5, 27, 3))
+    dy = rng.standard_normal((1, 4, 4, 4, 5))
+    assert np.isclose(np.sum(U.conv3(x, W, None) * dy), np.sum(x * U.conv3_dgrad(dy, W)), rtol=1e-12)
+    assert np.isclose(np.sum(U.conv3(x, W, None) * dy), np.sum(W * U.conv3_wgrad(dy, x)), rtol=1e-12)
+    p = rng.standard_normal((1, 2, 2, 2, 3))
+    assert np.isclose(np.sum(U.pool(x) * p), np.sum(x * U.pool_bwd(p)), rtol=1e-12)
+    assert np.isclose(np.sum(U.up(p) * x), np.sum(p * U.up_bwd(x)), rtol=1e-12)
+
+
+def test_oracle_gradients_match_finite_differences():
+    params = random_params(0)
+    x, t, m = random_problem(B=1, D=4, H=4, W=4, seed=2)
+    loss, g = U.loss_and_grads(params, x, t, m, bf=False)
+    rng = np.random.default_rng(5)
+    blocks, hw, hb, total = U.layout()
+    # a few weights and the bias of every layer, plus the head
+    idx = [w + int(rng.integers(27 * ci * co)) for (w, _), (ci, co) in zip(blocks, U.CONVS) if ci != 8]
+    idx += [w + 1 for (w, _), (ci, co) in zip(blocks, U.CONVS) if ci == 8]  # channel 1 of c1a
+    idx += [b for (_, b) in blocks] + [hw + 3, hb]
+    for i in idx:
+        eps = 1e-6
+        pp, pm = params.copy(), params.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        lp, _ = U.loss_and_grads(pp, x, t, m)
+        lm, _ = U.loss_and_grads(pm, x, t, m)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_bf16_rounding_is_nearest_even():
+    vals = np.array([1.0, 1.0 + 2**-8, 1.0 + 2**-7 + 2**-8, -3.14159, 0.0], np.float32)
+    got = U.bf16(vals)
+    assert got[0] == 1.0
+    assert got[1] == 1.0                      # tie -> even
+    assert got[2] == 1.0 + 2**-6              # tie -> even (up)
+    assert abs(got[3] + 3.140625) < 1e-12
+    assert got[4] == 0.0
+
+
+# ---------------------------------------------------------------- GPU: kernels against the oracle
+@pytest.mark.gpu
+def test_mfma_lane_layout():
+    from aquarius_amd.unet import mfma_probe
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((16, 32)).astype(np.float32)
+    B = rng.standard_normal((32, 16)).astype(np.float32)
+    D = mfma_probe(A, B)
+    want = U.bf16(A) @ U.bf16(B)
+    assert rel_l2(D, want) < 1e-6
+
+
+@pytest.mark.gpu
+def test_forward_matches_oracle():
+    from aquarius_amd.unet import UNet
+    net = UNet(seed=7)
+    params = net.get_params().astype(np.float64)
+    params[U.layout()[0][0][1]:U.layout()[0][0][1] + 16] = 0.05  # non-zero biases
+    net.set_params(params)
+    x, t, m = random_problem(B=2, D=16, H=8, W=12, seed=3)
+    y = net.forward(x)
+    want = U.forward(net.get_params(), x, bf=True)
+    err = rel_l2(y, want)
+    print(f"forward rel_l2 vs bf16 oracle {err:.2e}; vs fp64 oracle {rel_l2(y, U.forward(net.get_params(), x)):.2e}")
+    assert err < 1e-2
+
+
+@pytest.mark.gpu
+def test_gradients_match_oracle():
+    from aquarius_amd.unet import UNet
+    net = UNet(seed=11)
+    x, t, m = random_problem(B=2, D=8, H=8, W=16, seed=4)
+    loss, g = net.gradients(x, t, m)
+    p = net.get_params()
+    want_loss, want = U.loss_and_grads(p, x, t, m, bf=True)
+    assert abs(loss - want_loss) <= 1e-2 * want_loss
+    blocks, hw, hb, total = U.layout()
+    for l, ((w, b), (ci, co)) in enumerate(zip(blocks, U.CONVS)):
+        ew = rel_l2(g[w:w + 27 * ci * co], want[w:w + 27 * ci * co])
+        eb = rel_l2(g[b:b + co], want[b:b + co])
+        print(f"layer {l} ({ci}->{co}): dW rel_l2 {ew:.2e}, db rel_l2 {eb:.2e}")
+        assert ew < 3e-2 and eb < 3e-2, l
+    assert rel_l2(g[hw:], want[hw:]) < 3e-2
+
+
+@pytest.mark.gpu
+def test_gradient_descent_on_gpu_gradients_reduces_loss():
+    """Plain gradient descent driven by the GPU gradients (host-tensor path) lowers
+    the loss on a fixed batch: the gradients point downhill."""
+    from aquarius_amd.unet import UNet
+    net = UNet(seed=3)
+    x, t, m = random_problem(B=2, D=8, H=8, W=8, seed=6)
+    p = net.get_params()
+    losses = []
+    for _ in range(30):
+        loss, g = net.gradients(x, t, m)
+        p = p - 0.05 * g
+        net.set_params(p)
+        losses.append(loss)
+    assert losses[-1] < 0.9 * losses[0], losses
+
+
+@pytest.mark.gpu
+def test_device_dataset_and_adam_training():
+    """Pairs captured from FLIP handles on the device; Adam steps lower the loss."""
+    from aquarius_amd import warmstart
+    from aquarius_amd.unet import UNet
+    net = UNet(seed=5, lr=2e-3)
+    info = warmstart.generate(net, 12, n=16, seed=3, per_scene=3, max_step=20, tol=1e-6, accept=1e-5,
+                              cap=20000, log=lambda *a: None)
+    assert net.dataset_size == 12 and info["pairs"] == 12
+    before = net.eval(8, 4)
+    hist = net.train(80, 4, 8, seed=1)
+    print(f"train loss {hist[:5].mean():.3e} -> {hist[-5:].mean():.3e}; held-out {before:.3e} -> {net.eval(8, 4):.3e}")
+    assert np.isfinite(hist).all()
+    assert hist[-10:].mean() < 0.7 * hist[:10].mean()
