@@ -106,6 +106,10 @@ struct aq_flip {
   };
   std::vector<StepGraph> graphs;
   bool use_graphs = true;
+  // Multi-sweep launches (one launch for many Jacobi or diffusion sweeps).
+  bool multi = false;
+  bool list_valid = false;  // S.list matches the current active-block flags
+  Sweeps S{};
   std::vector<hipEvent_t> event_pool;
   std::vector<Mark> pending;
   size_t pool_used = 0;
@@ -290,6 +294,41 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   h->nparts_red = 512;
   AQ_TRY(h->allocT(&h->red_partial, (size_t)h->nparts_red * 16));
   AQ_TRY(h->allocT(&h->red_out, 16));
+  const int nblk = (nch + 255) / 256;
+  AQ_TRY(h->allocT(&h->S.list, (size_t)nblk));
+  AQ_TRY(h->allocT(&h->S.count, 1));
+  AQ_TRY(h->allocT(&h->S.bar, 2));
+  AQ_TRY(h->allocT(&h->S.err, 1));
+  if (h->multi) {
+    // co-resident capacity, capped where more blocks only add barrier arrivals
+    h->S.grid = std::min(multi_sweep_capacity(h->K), std::max(nblk, 1));
+    h->S.grid = std::min(h->S.grid, 1024);
+    if (h->S.grid < 1) h->multi = false;
+  }
+  return AQ_OK;
+}
+
+void refresh_rhs(aq_flip* h) {
+  launch_refresh_rhs(h->G, h->gr, h->stream);
+  h->list_valid = false;
+}
+
+void ensure_active_list(aq_flip* h) {
+  if (h->list_valid) return;
+  launch_compact_active(h->G, h->gr, h->S, h->stream);
+  h->list_valid = true;
+}
+
+// After a synchronisation: a timed-out barrier voids the results of the launch.
+aq_status check_sweeps(aq_flip* h) {
+  if (!h->multi) return AQ_OK;
+  int err = 0;
+  HIP_TRY(hipMemcpy(&err, h->S.err, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) {
+    HIP_TRY(hipMemset(h->S.err, 0, sizeof(int)));
+    HIP_TRY(hipMemset(h->S.bar, 0, 2 * sizeof(unsigned)));
+    return fail(AQ_EHIP, "multi-sweep grid barrier timed out; results of the call are void");
+  }
   return AQ_OK;
 }
 
@@ -341,11 +380,20 @@ void stage_transfer(aq_flip* h) {
 // a sweep whose output buffer already holds those zeros may skip AIR cells and
 // blocks without FLUID (the result is identical); otherwise it writes every cell.
 void jacobi_sweeps(aq_flip* h, int n) {
-  for (int i = 0; i < n; ++i) {
+  while (n > 0) {
+    // Several sweeps in one launch once both buffers hold 0 on AIR cells (every
+    // sweep then skips AIR cells and inactive blocks, as the single launches do).
+    if (h->multi && n >= 2 && h->air_clean[0] && h->air_clean[1]) {
+      ensure_active_list(h);
+      launch_jacobi_multi(h->G, h->gr, h->pb, n, h->S, h->stream);
+      if (n & 1) h->pb ^= 1;
+      return;
+    }
     const int out = h->pb ^ 1;
     launch_jacobi(h->G, h->gr, h->pb, h->air_clean[out] ? 0 : 1, h->stream);
     h->air_clean[out] = true;
     h->pb = out;
+    --n;
   }
 }
 
@@ -377,8 +425,18 @@ void stage_diffusion(aq_flip* h, bool inStep) {
   if (iters <= 0) return;
   if (!inStep || h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
   if (inStep && h->K == 1) return;
-  for (int i = 0; i < iters; ++i) {  // the first sweep also copies AIR cells across
-    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, h->stream);
+  // The first sweep also copies AIR cells across; after it both buffers agree on AIR.
+  launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, 1, h->stream);
+  h->nf ^= 1;
+  int left = iters - 1;
+  if (h->multi && left >= 2) {
+    ensure_active_list(h);
+    launch_diffusion_multi(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, left, h->S, h->stream);
+    if (left & 1) h->nf ^= 1;
+    left = 0;
+  }
+  for (; left > 0; --left) {
+    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, 0, h->stream);
     h->nf ^= 1;
   }
 }
@@ -390,11 +448,12 @@ aq_status run_stage(aq_flip* h, int st) {
     case AQ_STAGE_ADVECT: launch_advect(h->G, h->P[h->cur], h->Np, h->gr, h->nv, h->sp, h->stream); break;
     case AQ_STAGE_TRANSFER_TO_GRID:
       stage_transfer(h);
-      launch_refresh_rhs(h->G, h->gr, h->stream);  // content changed: re-mark AIR for Jacobi
+      refresh_rhs(h);  // content changed: re-mark AIR for Jacobi
       break;
     case AQ_STAGE_GRAVITY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 1, 0, h->stream); break;
     case AQ_STAGE_BOUNDARY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 0, 1, h->stream); break;
-    case AQ_STAGE_DIVERGENCE: launch_divergence(h->G, h->gr, h->nv, h->B, h->stream); break;
+    case AQ_STAGE_DIVERGENCE: launch_divergence(h->G, h->gr, h->nv, h->B, h->stream);
+  h->list_valid = false; break;
     case AQ_STAGE_PRESSURE: stage_pressure(h); break;
     case AQ_STAGE_PROJECT: launch_project(h->G, h->gr, h->nv, h->hv, h->pb, h->stream); break;
     case AQ_STAGE_EXTRAPOLATE: stage_extrapolate(h); break;
@@ -412,6 +471,7 @@ aq_status residual_now(aq_flip* h, double* rel) {
   double o[2];
   HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  AQ_TRY(check_sweeps(h));
   *rel = o[1] > 0 ? std::sqrt(o[0]) / std::sqrt(o[1]) : std::sqrt(o[0]);
   return AQ_OK;
 }
@@ -512,6 +572,7 @@ aq_status enqueue_step(aq_flip* h) {
   h->end(AQ_STAGE_GRAVITY);
   h->begin(AQ_STAGE_DIVERGENCE);
   launch_divergence(h->G, h->gr, h->nv, h->B, h->stream);
+  h->list_valid = false;
   h->end(AQ_STAGE_DIVERGENCE);
   AQ_TRY(pressure_stage(h));
   h->begin(AQ_STAGE_PROJECT);
@@ -583,6 +644,7 @@ aq_status create_common(const aq_flip_config* cfg, const float* pos, const float
                         const float* frac4, long long n, aq_flip** out) {
   aq_flip* h = new aq_flip;
   if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
+  if (const char* env = std::getenv("AQFLIP_MULTI_SWEEP")) h->multi = env[0] == '1';
   aq_status st = setup(h, cfg, n);
   if (st == AQ_OK) st = check_positions(h, pos, n);
   if (st == AQ_OK) {
@@ -593,7 +655,7 @@ aq_status create_common(const aq_flip_config* cfg, const float* pos, const float
     // Fluid::init: fixBoundary, then transferToGrid (Fluid_3D_FLIP.cu:485,500).
     launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 0, 1, h->stream);
     stage_transfer(h);
-    launch_refresh_rhs(h->G, h->gr, h->stream);
+    refresh_rhs(h);
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) st = fail(AQ_EHIP, std::string("init: ") + hipGetErrorString(e));
@@ -756,6 +818,7 @@ aq_status aq_flip_steps(aq_flip* h, int32_t count) {
   if (!h || count < 0) return fail(AQ_EINVAL, "bad argument");
   for (int i = 0; i < count; ++i) AQ_TRY(graph_step(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  AQ_TRY(check_sweeps(h));
   return h->collect();
 }
 
@@ -764,6 +827,7 @@ aq_status aq_flip_run_stage(aq_flip* h, int32_t stage) {
   if (stage < 0 || stage >= AQ_STAGE_COUNT) return fail(AQ_EINVAL, "unknown stage");
   AQ_TRY(run_stage(h, stage));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  AQ_TRY(check_sweeps(h));
   return h->collect();
 }
 
@@ -927,7 +991,7 @@ aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t b
       each([&](size_t o, size_t g) { c[g] = (uint8_t)I[o]; });
       AQ_TRY(h2d(h, h->gr.content, c));
       content_changed(h);
-      launch_refresh_rhs(h->G, h->gr, h->stream);
+      refresh_rhs(h);
       HIP_TRY(hipStreamSynchronize(h->stream));
       break;
     }
@@ -955,7 +1019,7 @@ aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t b
       if (field == AQ_FIELD_PRESSURE) h->air_clean[h->pb] = false;
       if (field != AQ_FIELD_PRESSURE) {
         // keep the Jacobi right-hand side consistent with divergence*density
-        launch_refresh_rhs(h->G, h->gr, h->stream);
+        refresh_rhs(h);
         HIP_TRY(hipStreamSynchronize(h->stream));
       }
       break;
@@ -1033,6 +1097,7 @@ aq_status aq_flip_residual_fp64(aq_flip* h, double* rel) {
   double o[2];
   HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  AQ_TRY(check_sweeps(h));
   *rel = o[1] > 0 ? std::sqrt(o[0]) / std::sqrt(o[1]) : std::sqrt(o[0]);
   return AQ_OK;
 }

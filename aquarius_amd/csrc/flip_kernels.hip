@@ -7,6 +7,8 @@
 // reference's racy in-place texture/surface update.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "aqflip_internal.h"
 
 #pragma clang fp contract(off)
@@ -297,7 +299,12 @@ void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, lon
 // z factors and accumulates its range in order, so every sum sees the reference's
 // operands in the reference's order.
 constexpr int kP2GWave = 64;
-constexpr int kP2GChunk = 256;  // particles staged per pass
+constexpr int kP2GChunk = 128;  // particles staged per pass (LDS bounds occupancy)
+// Staged particle k lives at LDS slot k + k/8. Lanes read particles about 8 apart
+// (8 per cell), which unpadded would put every lane's 16-byte read in the same
+// two bank groups; the skew spreads a 16-lane read phase over all 64 banks.
+constexpr int kP2GSlots = kP2GChunk + kP2GChunk / 8;
+__device__ __forceinline__ int p2g_slot(int k) { return k + (k >> 3); }
 
 struct P2GGeom {
   int nsegz;
@@ -308,9 +315,10 @@ template <int K>
 __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles P,
                                                   const int* __restrict__ off, Grid gr, int nv,
                                                   int hv, int nf, StepParams sp) {
-  __shared__ float4 sA[kP2GChunk];  // (kxF*kyC, kxC*kyF, kxC*kyC, pz)
-  __shared__ float4 sB[kP2GChunk];  // (vx, vy, vz, f0)
-  __shared__ float4 sC[kP2GChunk];  // (f1, f2, f3, -)
+  __shared__ float4 sA[kP2GSlots];                  // (kxF*kyC, kxC*kyF, kxC*kyC, pz)
+  __shared__ float4 sB[kP2GSlots];                  // (vx, vy, vz, f0)
+  __shared__ float sF1[K == 2 ? kP2GSlots : 1];     // f1 (two phases)
+  __shared__ float4 sC[K > 2 ? kP2GSlots : 1];      // (f1, f2, f3, -) (three or four phases)
   // block -> (x, y, z segment): b = (x*Y + y)*nsegz + zs
   const int b = xcd_block();
   const int xy = (int)fast_div((uint32_t)b, PG.divSeg);
@@ -356,23 +364,21 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
           const float4 B = P.b[cs + k];
           const float kxF = hat_div(A.x - xF, ih), kxC = hat_div(A.x - xC, ih);
           const float kyF = hat_div(A.y - yF, ih), kyC = hat_div(A.y - yC, ih);
-          sA[k] = make_float4(kxF * kyC, kxC * kyF, kxC * kyC, A.z);
-          sB[k] = make_float4(A.w, B.x, B.y, B.z);
-          if (K > 1) {
-            float4 c = make_float4(B.w, 0.f, 0.f, 0.f);
-            if (K > 2) {
-              const float4 Cc = P.c[cs + k];
-              c.y = Cc.x;
-              c.z = Cc.y;
-            }
-            sC[k] = c;
+          const int q = p2g_slot(k);
+          sA[q] = make_float4(kxF * kyC, kxC * kyF, kxC * kyC, A.z);
+          sB[q] = make_float4(A.w, B.x, B.y, B.z);
+          if (K == 2) sF1[q] = B.w;
+          if (K > 2) {
+            const float4 Cc = P.c[cs + k];
+            sC[q] = make_float4(B.w, Cc.x, Cc.y, 0.f);
           }
         }
         __syncthreads();
         const int j0 = max(lo, cs), j1 = min(hi, ce);
         for (int j = j0; j < j1; ++j) {
-          const float4 RA = sA[j - cs];
-          const float4 RB = sB[j - cs];
+          const int q = p2g_slot(j - cs);
+          const float4 RA = sA[q];
+          const float4 RB = sB[q];
           const float kzF = hat_div(RA.w - zF, ih), kzC = hat_div(RA.w - zC, ih);
           const float wx = RA.x * kzC;  // u face (x, y+.5, z+.5)
           const float wy = RA.y * kzC;  // v face (x+.5, y, z+.5)
@@ -386,10 +392,11 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
           if (centre && j >= ownLo && j < ownHi) {
             const float wc = RA.z * kzC;  // cell centre (x+.5, y+.5, z+.5)
             f[0] += wc * RB.w;
-            if (K > 1) {
-              const float4 RC = sC[j - cs];
+            if (K == 2) f[1] += wc * sF1[q];
+            if (K > 2) {
+              const float4 RC = sC[q];
               f[1] += wc * RC.x;
-              if (K > 2) f[2] += wc * RC.y;
+              f[2] += wc * RC.y;
               if (K > 3) f[3] += wc * RC.z;
             }
             ws += wc;
@@ -590,11 +597,9 @@ void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
 // otherwise (sum of the six clamp-to-edge neighbours + rhs) / 6, summed in the
 // reference order x+1, x-1, y+1, y-1, z+1, z-1, rhs.
 // AIR cells are recognised from the right-hand side's reserved bit pattern.
-__global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
-                         float* __restrict__ pout, const uint8_t* __restrict__ active, int full) {
-  if (!full && !active[xcd_block()]) return;
-  const int n = gtid();
-  if (n >= G.ncells_hash) return;
+__device__ __forceinline__ void jacobi_cell(const Geom& G, const float* __restrict__ rhs,
+                                            const float* __restrict__ pin, float* __restrict__ pout,
+                                            int n, int full) {
   int x, y, z;
   cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
@@ -616,9 +621,110 @@ __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __r
   s += b;
   pout[gi] = div_exact(s, 1.0 / 6.0);
 }
+
+__global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
+                         float* __restrict__ pout, const uint8_t* __restrict__ active, int full) {
+  if (!full && !active[xcd_block()]) return;
+  const int n = gtid();
+  if (n >= G.ncells_hash) return;
+  jacobi_cell(G, rhs, pin, pout, n, full);
+}
 void launch_jacobi(const Geom& G, Grid gr, int pin, int full, hipStream_t s) {
   hipLaunchKernelGGL(k_jacobi, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr.rhs,
                      gr.p[pin], gr.p[pin ^ 1], gr.active, full);
+}
+
+// ------------------------------------------------------------------ multi-sweep kernels
+// Active launch blocks (any FLUID cell) as an ascending list, so persistent sweeps
+// spread the work evenly. One block of kCompact threads, each owning a contiguous
+// run of flags.
+constexpr int kCompact = 1024;
+__global__ __launch_bounds__(kCompact) void k_compact_active(const uint8_t* __restrict__ active,
+                                                            int nb, int* __restrict__ list,
+                                                            int* __restrict__ count) {
+  __shared__ int sh[kCompact];
+  const int t = threadIdx.x;
+  const int per = (nb + kCompact - 1) / kCompact;
+  const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
+  int c = 0;
+  for (int b = b0; b < b1; ++b) c += active[b] ? 1 : 0;
+  sh[t] = c;
+  __syncthreads();
+  for (int o = 1; o < kCompact; o <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = t >= o ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  int w = sh[t] - c;
+  for (int b = b0; b < b1; ++b)
+    if (active[b]) list[w++] = b;
+  if (t == kCompact - 1) count[0] = sh[t];
+}
+void launch_compact_active(const Geom& G, Grid gr, Sweeps S, hipStream_t s) {
+  hipLaunchKernelGGL(k_compact_active, dim3(1), dim3(kCompact), 0, s, gr.active,
+                     nblocks(G.ncells_hash), S.list, S.count);
+}
+
+// Grid-wide barrier for a launch whose blocks are all resident at once (the grid
+// never exceeds the occupancy-derived capacity, and one stream runs one kernel at
+// a time). One thread per block publishes its block's writes (agent-scope release),
+// counts itself in and polls until `target` blocks have arrived, then acquires.
+// The poll is bounded: on time-out the launch records an error and carries on, so
+// every wave ends; the host reports the error and the results are void.
+// A barrier normally completes in microseconds; the limit (~10^5 polls) is three
+// orders of magnitude above that. Once any block has timed out, the others stop
+// waiting as soon as they see the error flag.
+constexpr int kSpinLimit = 1 << 16;
+__device__ __forceinline__ void grid_sync(unsigned* bar, unsigned target, int* err) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(bar, 1u);
+    int spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit) {
+        atomicOr(err, 1);
+        break;
+      }
+      if ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+// After its last barrier every block counts itself out; the last one resets both
+// counters for the next launch (nobody polls them any more by then).
+__device__ __forceinline__ void grid_done(unsigned* bar, unsigned* done) {
+  if (threadIdx.x == 0) {
+    if (atomicAdd(done, 1u) == gridDim.x - 1) {
+      atomicExch(bar, 0u);
+      atomicExch(done, 0u);
+    }
+  }
+}
+
+// `sweeps` Jacobi sweeps in one launch over the active blocks only (both buffers
+// must already hold 0 on AIR cells), one grid barrier between sweeps. Per-cell
+// arithmetic is jacobi_cell's, so results equal `sweeps` k_jacobi launches.
+__global__ __launch_bounds__(kBlock) void k_jacobi_multi(Geom G, const float* __restrict__ rhs,
+                                                         float* pa, float* pb, Sweeps S, int sweeps) {
+  const int na = S.count[0];
+  for (int it = 0; it < sweeps; ++it) {
+    const float* pin = (it & 1) ? pb : pa;
+    float* pout = (it & 1) ? pa : pb;
+    for (int t = blockIdx.x; t < na; t += gridDim.x) {
+      const int n = S.list[t] * kBlock + (int)threadIdx.x;
+      if (n < G.ncells_hash) jacobi_cell(G, rhs, pin, pout, n, 0);
+    }
+    if (it + 1 < sweeps) grid_sync(S.bar, (unsigned)(it + 1) * gridDim.x, S.err);
+  }
+  grid_done(S.bar, S.bar + 1);
+}
+void launch_jacobi_multi(const Geom& G, Grid gr, int pin, int sweeps, Sweeps S, hipStream_t s) {
+  hipLaunchKernelGGL(k_jacobi_multi, dim3(S.grid), dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin],
+                     gr.p[pin ^ 1], S, sweeps);
 }
 
 // diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
@@ -759,10 +865,8 @@ void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t 
 // f' = (lambda * sum of the six clamp-to-edge neighbours + f) / (1 + 2 lambda),
 // then normalised by its channel sum; AIR cells keep their value.
 template <int K>
-__global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda, double inv_cc, int full) {
-  if (!full && !gr.active[xcd_block()]) return;
-  const int n = gtid();
-  if (n >= G.ncells_hash) return;
+__device__ __forceinline__ void diffusion_cell(const Geom& G, const Grid& gr, int fin, float lambda,
+                                               double inv_cc, int n, int full) {
   int x, y, z;
   cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
@@ -795,6 +899,14 @@ __global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda, double inv_c
 #pragma unroll
   for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = r[a] / sum;
 }
+
+template <int K>
+__global__ void k_diffusion(Geom G, Grid gr, int fin, float lambda, double inv_cc, int full) {
+  if (!full && !gr.active[xcd_block()]) return;
+  const int n = gtid();
+  if (n >= G.ncells_hash) return;
+  diffusion_cell<K>(G, gr, fin, lambda, inv_cc, n, full);
+}
 void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc, int full,
                       hipStream_t s) {
   const int nb = nblocks(G.ncells_hash);
@@ -804,6 +916,47 @@ void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, doub
     case 3: hipLaunchKernelGGL(k_diffusion<3>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, full); break;
     default: hipLaunchKernelGGL(k_diffusion<4>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, full); break;
   }
+}
+
+// `sweeps` diffusion sweeps in one launch over the active blocks (both buffers must
+// already agree on AIR cells), one grid barrier between sweeps.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_diffusion_multi(Geom G, Grid gr, int fin, float lambda,
+                                                            double inv_cc, Sweeps S, int sweeps) {
+  const int na = S.count[0];
+  for (int it = 0; it < sweeps; ++it) {
+    const int f = fin ^ (it & 1);
+    for (int t = blockIdx.x; t < na; t += gridDim.x) {
+      const int n = S.list[t] * kBlock + (int)threadIdx.x;
+      if (n < G.ncells_hash) diffusion_cell<K>(G, gr, f, lambda, inv_cc, n, 0);
+    }
+    if (it + 1 < sweeps) grid_sync(S.bar, (unsigned)(it + 1) * gridDim.x, S.err);
+  }
+  grid_done(S.bar, S.bar + 1);
+}
+void launch_diffusion_multi(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc,
+                            int sweeps, Sweeps S, hipStream_t s) {
+  switch (K) {
+    case 1: hipLaunchKernelGGL(k_diffusion_multi<1>, dim3(S.grid), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, S, sweeps); break;
+    case 2: hipLaunchKernelGGL(k_diffusion_multi<2>, dim3(S.grid), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, S, sweeps); break;
+    case 3: hipLaunchKernelGGL(k_diffusion_multi<3>, dim3(S.grid), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, S, sweeps); break;
+    default: hipLaunchKernelGGL(k_diffusion_multi<4>, dim3(S.grid), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, S, sweeps); break;
+  }
+}
+// Largest grid of k_jacobi_multi / k_diffusion_multi<K> whose blocks fit on the
+// device at once.
+int multi_sweep_capacity(int K) {
+  int dev = 0, cus = 0, a = 0, b = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void*)k_jacobi_multi, kBlock, 0) != hipSuccess)
+    return 0;
+  const void* fd = K == 1 ? (const void*)k_diffusion_multi<1>
+                 : K == 2 ? (const void*)k_diffusion_multi<2>
+                 : K == 3 ? (const void*)k_diffusion_multi<3>
+                          : (const void*)k_diffusion_multi<4>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fd, kBlock, 0) != hipSuccess) return 0;
+  return cus * std::min(a, b);
 }
 
 // ------------------------------------------------------------------ G2P and advection

@@ -43,16 +43,19 @@ def random_scene(rng: np.random.Generator, n: int = 64, seed: int = 0):
     return cfg
 
 
-def converge(h: FlipHandle, tol: float, cap: int, check_every: int = 100):
-    """Re-solve the handle's current pressure system from zero with plain Jacobi."""
-    h.set_field("pressure", np.zeros(h.field_shape("pressure"), np.float32))
+def converge(h: FlipHandle, tol: float, cap: int, check_every: int = 100, start=None):
+    """Re-solve the handle's current pressure system with plain Jacobi, from zero or
+    from `start` (a pressure field of the same extent)."""
+    h.set_field("pressure", np.zeros(h.field_shape("pressure"), np.float32) if start is None else start)
     return h.solve_pressure(tol, cap, check_every)
 
 
 def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int = 4,
              max_step: int = 60, tol: float = 1e-5, accept: float = 1e-4, cap: int = 60000,
              log=print):
-    """Fill the network's device dataset with `pairs` converged systems."""
+    """Fill the network's device dataset with `pairs` converged systems. Within a
+    scene each system is solved starting from the previous stop's converged pressure
+    (the fixed point does not depend on the start, only the sweep count does)."""
     net.reserve(pairs, (n, n, n))
     rng = np.random.default_rng(seed)
     scene = 0
@@ -63,13 +66,15 @@ def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int =
         h = FlipHandle(cfg)
         stops = sorted(set(int(s) for s in rng.integers(1, max_step + 1, per_scene)))
         done = 0
+        prev = None
         for s in stops:
             h.steps(s - done)
             done = s
-            it, rel, _ = converge(h, tol, cap)
+            it, rel, _ = converge(h, tol, cap, start=prev)
             sweeps += it
             if rel <= accept and net.dataset_size < pairs:
                 net.add_sample(h)
+                prev = h.get_field("pressure")
         h.close()
         scene += 1
         if scene % 50 == 0:
@@ -107,6 +112,22 @@ def evaluate(net: UNet, n: int = 64, scenes: int = 8, seed: int = 12345, steps: 
                 ratio=float(p.sum() / max(q.sum(), 1)))
 
 
+def evaluate_bench(net: UNet, stops=(1, 10, 30), tol: float = 1e-4, log=print):
+    """Sweeps to `tol` on the 128^3 multiphase scene after `stops` steps."""
+    h = FlipHandle(configs.multiphase_128())
+    rows = []
+    done = 0
+    for s in stops:
+        h.steps(s - done)
+        done = s
+        rows.append(dict(step=s, **compare_on(h, net, tol, cap=200000, check_every=50)))
+        log("  ", rows[-1])
+    h.close()
+    p = sum(r["plain_iters"] for r in rows)
+    q = sum(r["nn_iters"] for r in rows)
+    return dict(rows=rows, ratio=p / max(q, 1))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -120,17 +141,20 @@ def main(argv=None) -> int:
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--out", type=Path, default=WEIGHTS)
     t.add_argument("--eval-scenes", type=int, default=8)
+    t.add_argument("--tol", type=float, default=1e-5)
+    t.add_argument("--eval-128", action="store_true")
     c = sub.add_parser("compare")
     c.add_argument("--weights", type=Path, default=WEIGHTS)
     c.add_argument("--n", type=int, default=64)
     c.add_argument("--scenes", type=int, default=4)
+    c.add_argument("--bench", action="store_true")
     a = ap.parse_args(argv)
 
     if a.cmd == "train":
         net = UNet(lr=a.lr, seed=a.seed + 1)
         total = a.pairs + a.held_out
         print(f"[warmstart] generating {total} pairs at {a.n}^3", flush=True)
-        gen = generate(net, total, n=a.n, seed=a.seed)
+        gen = generate(net, total, n=a.n, seed=a.seed, tol=a.tol)
         print(f"[warmstart] data: {json.dumps(gen)}", flush=True)
         train_n = a.pairs
         l0 = net.eval(train_n, a.held_out)
@@ -150,11 +174,14 @@ def main(argv=None) -> int:
         ev = evaluate(net, n=a.n, scenes=a.eval_scenes)
         print(f"[warmstart] sweeps to 1e-4: plain {ev['plain_mean']:.0f}, warm {ev['nn_mean']:.0f}, "
               f"ratio {ev['ratio']:.2f}", flush=True)
+        if a.eval_128:
+            eb = evaluate_bench(net)
+            print(f"[warmstart] 128^3 multiphase: sweep ratio {eb['ratio']:.2f}", flush=True)
         return 0
     if a.cmd == "compare":
         net = UNet()
         net.load_file(a.weights)
-        ev = evaluate(net, n=a.n, scenes=a.scenes)
+        ev = evaluate_bench(net) if a.bench else evaluate(net, n=a.n, scenes=a.scenes)
         print(json.dumps({k: v for k, v in ev.items() if k != "rows"}), flush=True)
         return 0
     return 1

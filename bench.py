@@ -13,7 +13,11 @@ Prints one JSON line with the contract fields plus:
                 8 TB/s HBM peak (per-launch duration from HIP events recorded on
                 the library's own stream over the timed region)
   kernels       the same figure for every stage of the step
-  jacobi_to_tol plain-Jacobi sweeps from zero to a 1e-4 relative FP64 residual
+  jacobi_to_tol plain-Jacobi sweeps from zero to a 1e-4 relative FP64 residual on
+                the last step's system, and (with the trained U-Net weights present)
+                the sweeps from the U-Net guess to the same residual
+  tolerance_mode steps/s when every step solves to 1e-4: plain Jacobi from zero
+                against the U-Net warm start
   cpu_baseline  the CPU oracle timed on a bounded sample of the same workload
 """
 from __future__ import annotations
@@ -60,6 +64,79 @@ def launches_per_step(cfg, K: int) -> dict:
             "diffusion": cfg.diffusion_iters if K > 1 else 0}
 
 
+# Kernels of each stage, as rocprofv3 names them (template arguments stripped).
+STAGE_KERNELS = {
+    "g2p": ["aqf::k_g2p_advect_hash"],
+    "transfer_to_grid": ["aqf::k_fill_u32", "aqf::k_scan_reduce", "aqf::k_scan_partials",
+                         "aqf::k_scan_final", "aqf::k_scatter", "aqf::k_segsort", "aqf::k_gather",
+                         "aqf::k_p2g"],
+    "gravity": ["aqf::k_gravity_boundary"],
+    "divergence": ["aqf::k_divergence"],
+    "pressure": ["aqf::k_jacobi"],
+    "project": ["aqf::k_project"],
+    "extrapolate": ["aqf::k_extrapolate"],
+    "diffusion": ["aqf::k_diffusion"],
+}
+
+
+def measured_traffic(scene: str, stage: str):
+    """HBM bytes per launch of `stage` from the committed FETCH_SIZE/WRITE_SIZE
+    summary of this workload (copy-calibrated), or None when there is none."""
+    path = ROOT / "profiles" / f"hbm_traffic_{scene}.md"
+    if not path.exists():
+        return None
+    text = path.read_text()
+    start = text.find("```json")
+    if start < 0:
+        return None
+    data = json.loads(text[start + 7:text.find("```", start + 7)])
+    total, found = 0, False
+    for name, row in data["kernels"].items():
+        base = name.split("<")[0]
+        if base in STAGE_KERNELS.get(stage, []):
+            total += row["fetch_bytes"] + row["write_bytes"]
+            found = True
+    return total if found else None
+
+
+def load_warmstart_net():
+    """The trained warm-start U-Net shipped with the package, or None."""
+    from aquarius_amd import warmstart
+    if not warmstart.WEIGHTS.exists():
+        return None
+    from aquarius_amd.unet import UNet
+    net = UNet()
+    net.load_file(warmstart.WEIGHTS)
+    return net
+
+
+def tolerance_mode_steps(cfg, steps: int, cap: int):
+    """Steps/s with every step's pressure solved to a 1e-4 FP64 relative residual:
+    plain Jacobi from zero against the U-Net warm start (zero-guess fallback), on two
+    fresh handles of the same workload."""
+    from aquarius_amd.fluid import FlipHandle
+    from aquarius_amd.unet import attach_solver, solver_stats
+    net = load_warmstart_net()
+    modes = [("plain", "tol_zero", None)] + ([("nn", "tol_nn", net)] if net is not None else [])
+    out = {"tol": 1e-4, "steps": steps, "check_every": 50}
+    for label, mode, nnet in modes:
+        h = FlipHandle(cfg)
+        attach_solver(h, mode, tol=1e-4, max_iters=cap, check_every=50, net=nnet)
+        h.steps(1)  # warm-up (first solve, lazy allocations)
+        solver_stats(h, reset=True)
+        t0 = time.perf_counter()
+        h.steps(steps)
+        secs = time.perf_counter() - t0
+        st = solver_stats(h)
+        h.close()
+        out[label] = {"steps_per_s": round(steps / secs, 3), "mean_sweeps": st["sweeps"] / max(st["solves"], 1),
+                      "fallbacks": st["fallbacks"], "capped": st["capped"]}
+    if "nn" in out:
+        out["speedup"] = round(out["nn"]["steps_per_s"] / out["plain"]["steps_per_s"], 3)
+        out["sweep_ratio"] = round(out["plain"]["mean_sweeps"] / max(out["nn"]["mean_sweeps"], 1), 3)
+    return out
+
+
 def cpu_baseline(scene: str, steps: int) -> dict:
     from aquarius_amd import configs
     from oracle import binding as ob
@@ -85,6 +162,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-cap", type=int, default=60000)
+    ap.add_argument("--tol-steps", type=int, default=10,
+                    help="steps per solver mode in the tolerance-mode comparison (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -140,7 +219,7 @@ def main() -> int:
     dk = kernels[dominant]
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": dk["achieved_GBs"],
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["frac_of_hbm_peak"],
-                "traffic": None}
+                "traffic": measured_traffic(args.scene, dominant)}
 
     solve = None
     if not args.no_solve:
@@ -151,7 +230,21 @@ def main() -> int:
         solve = {"tol": 1e-4, "plain_iters": it, "plain_residual": rel, "plain_converged": ok,
                  "plain_seconds": round(time.perf_counter() - ts, 3), "cap": args.solve_cap,
                  "nn_warm_iters": None}
+        net = load_warmstart_net()
+        if net is not None:
+            from aquarius_amd.unet import nn_guess
+            nn_guess(h, net)
+            g = h.residual()
+            ts = time.perf_counter()
+            it2, rel2, ok2 = h.solve_pressure(1e-4, args.solve_cap, check_every=10)
+            solve.update({"nn_warm_iters": it2, "nn_guess_residual": g, "nn_residual": rel2,
+                          "nn_converged": ok2, "nn_seconds": round(time.perf_counter() - ts, 3),
+                          "iter_ratio": round(it / max(it2, 1), 3)})
     h.close()
+
+    tol_mode = None
+    if not args.no_solve and args.tol_steps > 0:
+        tol_mode = tolerance_mode_steps(cfg, args.tol_steps, args.solve_cap)
 
     out = {
         "metric": METRIC,
@@ -174,6 +267,7 @@ def main() -> int:
         "roofline": roofline,
         "kernels": kernels,
         "jacobi_to_tol": solve,
+        "tolerance_mode": tol_mode,
         "profiled_pass_ms_per_step": round(eager_ms_per_step, 4),
     }
     if not args.no_cpu_baseline:
