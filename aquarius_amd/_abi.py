@@ -99,6 +99,7 @@ class UNetConfig(C.Structure):
         ("beta2", C.c_float),
         ("eps", C.c_float),
         ("target_scale", C.c_float),
+        ("infer_extent", C.c_int32),
     ]
 
 
@@ -109,6 +110,7 @@ class Solver(C.Structure):
         ("max_iters", C.c_int32),
         ("check_every", C.c_int32),
         ("stall_checks", C.c_int32),
+        ("nn_refresh", C.c_int32),
     ]
 
 
@@ -122,6 +124,7 @@ class SolverStats(C.Structure):
         ("last_residual", C.c_double),
         ("last_guess_residual", C.c_double),
         ("last_fallback", C.c_int32),
+        ("nn_corrections", C.c_int64),
     ]
 
 
@@ -175,6 +178,7 @@ PROTOTYPES = {
     "aq_unet_train": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, _FP]),
     "aq_unet_eval": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "aq_flip_nn_guess": (C.c_int, [_P, _P]),
+    "aq_flip_nn_correct": (C.c_int, [_P, _P]),
     "aq_flip_set_solver": (C.c_int, [_P, C.POINTER(Solver), _P]),
     "aq_flip_solver_stats_get": (C.c_int, [_P, C.POINTER(SolverStats), C.c_int32]),
     "aq_unet_mfma_probe": (C.c_int, [_FP, _FP, _FP]),

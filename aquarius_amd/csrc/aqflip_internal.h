@@ -83,7 +83,8 @@ struct Grid {
   float* density;
   float* vf[4];         // volumeFractions
   float* nvf[2][4];     // newVolumeFractions, ping-pong
-  uint8_t* active;      // per 256-cell launch block of the interior: any FLUID cell
+  uint32_t* active;     // per 256-cell launch block of the interior: any FLUID cell
+                        // (32-bit so a block reads its flag with one scalar load)
 };
 
 struct Bins {
@@ -93,15 +94,6 @@ struct Bins {
   int* hash;      // per particle, previous order
   int* rank;      // per particle: slot within its cell's segment (pre-sort)
   int* perm;      // sorted slot -> previous index
-};
-
-// Multi-sweep launches: the active-block list and the grid barrier's state.
-struct Sweeps {
-  int* list;      // active launch-block ids, ascending
-  int* count;     // number of entries in list
-  unsigned* bar;  // [0] barrier arrivals, [1] blocks finished (both 0 between launches)
-  int* err;       // set to 1 if a barrier wait timed out
-  int grid;       // blocks per launch (<= co-resident capacity)
 };
 
 struct StepParams {
@@ -129,12 +121,6 @@ void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 // full = 1: every cell written (AIR cells get 0); full = 0: blocks without FLUID
 // cells and AIR cells are skipped -- valid once both buffers hold 0 on AIR.
 void launch_jacobi(const Geom& G, Grid gr, int pin, int full, hipStream_t s);
-// Multi-sweep variants: `sweeps` non-full sweeps in one launch, active blocks only.
-void launch_compact_active(const Geom& G, Grid gr, Sweeps S, hipStream_t s);
-void launch_jacobi_multi(const Geom& G, Grid gr, int pin, int sweeps, Sweeps S, hipStream_t s);
-void launch_diffusion_multi(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc,
-                            int sweeps, Sweeps S, hipStream_t s);
-int multi_sweep_capacity(int K);
 void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s);
 void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s);

@@ -83,7 +83,7 @@ struct aq_flip {
   float physicalTime = 0.f;
 
   // Pressure solve mode (fixed sweeps as the reference, or to a tolerance).
-  aq_flip_solver solver{AQ_SOLVE_FIXED, 1e-4, 100000, 50, 20};
+  aq_flip_solver solver{AQ_SOLVE_FIXED, 1e-4, 100000, 50, 20, 0};
   aq_unet* net = nullptr;
   aq_flip_solver_stats sstats{};
 
@@ -106,10 +106,6 @@ struct aq_flip {
   };
   std::vector<StepGraph> graphs;
   bool use_graphs = true;
-  // Multi-sweep launches (one launch for many Jacobi or diffusion sweeps).
-  bool multi = false;
-  bool list_valid = false;  // S.list matches the current active-block flags
-  Sweeps S{};
   std::vector<hipEvent_t> event_pool;
   std::vector<Mark> pending;
   size_t pool_used = 0;
@@ -294,43 +290,10 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   h->nparts_red = 512;
   AQ_TRY(h->allocT(&h->red_partial, (size_t)h->nparts_red * 16));
   AQ_TRY(h->allocT(&h->red_out, 16));
-  const int nblk = (nch + 255) / 256;
-  AQ_TRY(h->allocT(&h->S.list, (size_t)nblk));
-  AQ_TRY(h->allocT(&h->S.count, 1));
-  AQ_TRY(h->allocT(&h->S.bar, 2));
-  AQ_TRY(h->allocT(&h->S.err, 1));
-  if (h->multi) {
-    // co-resident capacity, capped where more blocks only add barrier arrivals
-    h->S.grid = std::min(multi_sweep_capacity(h->K), std::max(nblk, 1));
-    h->S.grid = std::min(h->S.grid, 1024);
-    if (h->S.grid < 1) h->multi = false;
-  }
   return AQ_OK;
 }
 
-void refresh_rhs(aq_flip* h) {
-  launch_refresh_rhs(h->G, h->gr, h->stream);
-  h->list_valid = false;
-}
-
-void ensure_active_list(aq_flip* h) {
-  if (h->list_valid) return;
-  launch_compact_active(h->G, h->gr, h->S, h->stream);
-  h->list_valid = true;
-}
-
-// After a synchronisation: a timed-out barrier voids the results of the launch.
-aq_status check_sweeps(aq_flip* h) {
-  if (!h->multi) return AQ_OK;
-  int err = 0;
-  HIP_TRY(hipMemcpy(&err, h->S.err, sizeof(int), hipMemcpyDeviceToHost));
-  if (err) {
-    HIP_TRY(hipMemset(h->S.err, 0, sizeof(int)));
-    HIP_TRY(hipMemset(h->S.bar, 0, 2 * sizeof(unsigned)));
-    return fail(AQ_EHIP, "multi-sweep grid barrier timed out; results of the call are void");
-  }
-  return AQ_OK;
-}
+void refresh_rhs(aq_flip* h) { launch_refresh_rhs(h->G, h->gr, h->stream); }
 
 aq_status upload_particles(aq_flip* h, const float* pos, const float* vel, const float* frac4) {
   const long long n = h->Np;
@@ -380,20 +343,11 @@ void stage_transfer(aq_flip* h) {
 // a sweep whose output buffer already holds those zeros may skip AIR cells and
 // blocks without FLUID (the result is identical); otherwise it writes every cell.
 void jacobi_sweeps(aq_flip* h, int n) {
-  while (n > 0) {
-    // Several sweeps in one launch once both buffers hold 0 on AIR cells (every
-    // sweep then skips AIR cells and inactive blocks, as the single launches do).
-    if (h->multi && n >= 2 && h->air_clean[0] && h->air_clean[1]) {
-      ensure_active_list(h);
-      launch_jacobi_multi(h->G, h->gr, h->pb, n, h->S, h->stream);
-      if (n & 1) h->pb ^= 1;
-      return;
-    }
+  for (int i = 0; i < n; ++i) {
     const int out = h->pb ^ 1;
     launch_jacobi(h->G, h->gr, h->pb, h->air_clean[out] ? 0 : 1, h->stream);
     h->air_clean[out] = true;
     h->pb = out;
-    --n;
   }
 }
 
@@ -425,18 +379,8 @@ void stage_diffusion(aq_flip* h, bool inStep) {
   if (iters <= 0) return;
   if (!inStep || h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
   if (inStep && h->K == 1) return;
-  // The first sweep also copies AIR cells across; after it both buffers agree on AIR.
-  launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, 1, h->stream);
-  h->nf ^= 1;
-  int left = iters - 1;
-  if (h->multi && left >= 2) {
-    ensure_active_list(h);
-    launch_diffusion_multi(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, left, h->S, h->stream);
-    if (left & 1) h->nf ^= 1;
-    left = 0;
-  }
-  for (; left > 0; --left) {
-    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, 0, h->stream);
+  for (int i = 0; i < iters; ++i) {  // the first sweep also copies AIR cells across
+    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, h->stream);
     h->nf ^= 1;
   }
 }
@@ -452,8 +396,7 @@ aq_status run_stage(aq_flip* h, int st) {
       break;
     case AQ_STAGE_GRAVITY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 1, 0, h->stream); break;
     case AQ_STAGE_BOUNDARY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 0, 1, h->stream); break;
-    case AQ_STAGE_DIVERGENCE: launch_divergence(h->G, h->gr, h->nv, h->B, h->stream);
-  h->list_valid = false; break;
+    case AQ_STAGE_DIVERGENCE: launch_divergence(h->G, h->gr, h->nv, h->B, h->stream); break;
     case AQ_STAGE_PRESSURE: stage_pressure(h); break;
     case AQ_STAGE_PROJECT: launch_project(h->G, h->gr, h->nv, h->hv, h->pb, h->stream); break;
     case AQ_STAGE_EXTRAPOLATE: stage_extrapolate(h); break;
@@ -471,7 +414,6 @@ aq_status residual_now(aq_flip* h, double* rel) {
   double o[2];
   HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  AQ_TRY(check_sweeps(h));
   *rel = o[1] > 0 ? std::sqrt(o[0]) / std::sqrt(o[1]) : std::sqrt(o[0]);
   return AQ_OK;
 }
@@ -529,19 +471,28 @@ aq_status pressure_stage(aq_flip* h) {
     AQ_TRY(aq_flip_nn_guess(h, h->net));
     h->air_clean[h->pb] = true;  // the guess is 0 on AIR cells
     AQ_TRY(residual_now(h, &guess));
-    // A non-finite guess is rejected outright. A finite one is kept even when its
-    // residual exceeds zero's (1): its error is mostly high-frequency, which the first
-    // sweeps remove; a guess that then stalls falls back below.
+    // Jacobi converges to the same fixed point from any finite start, so only a
+    // non-finite guess or refinement is rejected for the zero-guess solve. A finite
+    // guess that stalls has reached FP32 Jacobi's round-off floor, where a solve from
+    // zero would stall as well.
     if (!std::isfinite(guess)) fallback = true;
-    if (!fallback) {
-      AQ_TRY(sweep_to_tol(h, h->solver.max_iters, true, &sweeps, &res, &stalled));
+    // Refine with Jacobi; every nn_refresh sweeps (if set) the network corrects the
+    // iterate from its residual, then Jacobi continues.
+    int left = h->solver.max_iters;
+    while (!fallback) {
+      const int budget = h->solver.nn_refresh > 0 ? std::min(h->solver.nn_refresh, left) : left;
+      AQ_TRY(sweep_to_tol(h, budget, true, &sweeps, &res, &stalled));
       total += sweeps;
-      if (stalled && !(res <= h->solver.tol)) fallback = true;
+      left -= sweeps;
+      if (!std::isfinite(res)) fallback = true;
+      if (fallback || res <= h->solver.tol || left <= 0 || stalled || h->solver.nn_refresh <= 0) break;
+      AQ_TRY(aq_flip_nn_correct(h, h->net));
+      st.nn_corrections += 1;
     }
   }
   if (h->solver.mode == AQ_SOLVE_TOL_ZERO || fallback) {
     zero_pressure(h);
-    AQ_TRY(sweep_to_tol(h, std::max(0, h->solver.max_iters), false, &sweeps, &res, &stalled));
+    AQ_TRY(sweep_to_tol(h, std::max(0, h->solver.max_iters), true, &sweeps, &res, &stalled));
     total += sweeps;
     if (h->solver.mode == AQ_SOLVE_TOL_ZERO) guess = 1.0;
   }
@@ -572,7 +523,6 @@ aq_status enqueue_step(aq_flip* h) {
   h->end(AQ_STAGE_GRAVITY);
   h->begin(AQ_STAGE_DIVERGENCE);
   launch_divergence(h->G, h->gr, h->nv, h->B, h->stream);
-  h->list_valid = false;
   h->end(AQ_STAGE_DIVERGENCE);
   AQ_TRY(pressure_stage(h));
   h->begin(AQ_STAGE_PROJECT);
@@ -644,7 +594,6 @@ aq_status create_common(const aq_flip_config* cfg, const float* pos, const float
                         const float* frac4, long long n, aq_flip** out) {
   aq_flip* h = new aq_flip;
   if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
-  if (const char* env = std::getenv("AQFLIP_MULTI_SWEEP")) h->multi = env[0] == '1';
   aq_status st = setup(h, cfg, n);
   if (st == AQ_OK) st = check_positions(h, pos, n);
   if (st == AQ_OK) {
@@ -714,8 +663,10 @@ extern "C" {
 aq_status aq_flip_set_solver(aq_flip* h, const aq_flip_solver* s, aq_unet* net) {
   if (!h || !s) return fail(AQ_EINVAL, "null argument");
   if (s->mode < AQ_SOLVE_FIXED || s->mode > AQ_SOLVE_TOL_NN) return fail(AQ_EINVAL, "unknown solver mode");
-  if (s->mode != AQ_SOLVE_FIXED && (!(s->tol > 0) || s->max_iters < 0 || s->check_every < 1 || s->stall_checks < 1))
-    return fail(AQ_EINVAL, "tolerance solves need tol > 0, max_iters >= 0, check_every >= 1, stall_checks >= 1");
+  if (s->mode != AQ_SOLVE_FIXED && (!(s->tol > 0) || s->max_iters < 0 || s->check_every < 1 ||
+                                    s->stall_checks < 1 || s->nn_refresh < 0))
+    return fail(AQ_EINVAL, "tolerance solves need tol > 0, max_iters >= 0, check_every >= 1, "
+                           "stall_checks >= 1, nn_refresh >= 0");
   if (s->mode == AQ_SOLVE_TOL_NN && !net) return fail(AQ_EINVAL, "AQ_SOLVE_TOL_NN needs a U-Net");
   if (s->mode == AQ_SOLVE_TOL_NN && (h->G.X % 4 || h->G.Y % 4 || h->G.Z % 4))
     return fail(AQ_EINVAL, "the U-Net needs grid extents that are multiples of 4");
@@ -818,7 +769,6 @@ aq_status aq_flip_steps(aq_flip* h, int32_t count) {
   if (!h || count < 0) return fail(AQ_EINVAL, "bad argument");
   for (int i = 0; i < count; ++i) AQ_TRY(graph_step(h));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  AQ_TRY(check_sweeps(h));
   return h->collect();
 }
 
@@ -827,7 +777,6 @@ aq_status aq_flip_run_stage(aq_flip* h, int32_t stage) {
   if (stage < 0 || stage >= AQ_STAGE_COUNT) return fail(AQ_EINVAL, "unknown stage");
   AQ_TRY(run_stage(h, stage));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  AQ_TRY(check_sweeps(h));
   return h->collect();
 }
 
@@ -1097,7 +1046,6 @@ aq_status aq_flip_residual_fp64(aq_flip* h, double* rel) {
   double o[2];
   HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  AQ_TRY(check_sweeps(h));
   *rel = o[1] > 0 ? std::sqrt(o[0]) / std::sqrt(o[1]) : std::sqrt(o[0]);
   return AQ_OK;
 }

@@ -7,8 +7,6 @@
 // reference's racy in-place texture/surface update.
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
-
 #include "aqflip_internal.h"
 
 #pragma clang fp contract(off)
@@ -297,9 +295,13 @@ void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, lon
 // particle are the same for the whole wave and are multiplied once, as
 // (kx * ky), the first product of trilinearHatKernel. Each lane then forms the
 // z factors and accumulates its range in order, so every sum sees the reference's
-// operands in the reference's order.
+// operands in the reference's order. Outside the centre slab (dx, dy) = (0, 0) about
+// half of the staged particles have x/y factor products that are all exactly zero;
+// they only add +-0 to sums that start at +0, which changes no bit for finite
+// velocities, so staging drops them (wave ballot, order kept) and each lane maps
+// its range onto the kept particles.
 constexpr int kP2GWave = 64;
-constexpr int kP2GChunk = 128;  // particles staged per pass (LDS bounds occupancy)
+constexpr int kP2GChunk = 256;  // particles staged per pass
 // Staged particle k lives at LDS slot k + k/8. Lanes read particles about 8 apart
 // (8 per cell), which unpadded would put every lane's 16-byte read in the same
 // two bank groups; the skew spreads a 16-lane read phase over all 64 banks.
@@ -358,25 +360,56 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
       const bool centre = (dx == 0) & (dy == 0);
       for (int cs = wlo; cs < whi; cs += kP2GChunk) {
         const int ce = min(cs + kP2GChunk, whi);
+        const int nst = ce - cs;
         __syncthreads();  // previous chunk fully consumed
-        for (int k = lane; k < ce - cs; k += kP2GWave) {
-          const float4 A = P.a[cs + k];
-          const float4 B = P.b[cs + k];
-          const float kxF = hat_div(A.x - xF, ih), kxC = hat_div(A.x - xC, ih);
-          const float kyF = hat_div(A.y - yF, ih), kyC = hat_div(A.y - yC, ih);
-          const int q = p2g_slot(k);
-          sA[q] = make_float4(kxF * kyC, kxC * kyF, kxC * kyC, A.z);
-          sB[q] = make_float4(A.w, B.x, B.y, B.z);
-          if (K == 2) sF1[q] = B.w;
-          if (K > 2) {
-            const float4 Cc = P.c[cs + k];
-            sC[q] = make_float4(B.w, Cc.x, Cc.y, 0.f);
+        // Stage the chunk; outside the centre slab keep only particles with a
+        // non-zero x/y factor product (the others add exact zeros to every sum).
+        uint64_t kept[kP2GChunk / kP2GWave];
+        int nkept = 0;
+#pragma unroll
+        for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
+          const int k = bi * kP2GWave + lane;
+          bool keep = false;
+          float4 A = {}, B = {}, Cc = {};
+          float pu = 0.f, pv = 0.f, pw = 0.f;
+          if (k < nst) {
+            A = P.a[cs + k];
+            B = P.b[cs + k];
+            if (K > 2) Cc = P.c[cs + k];
+            const float kxF = hat_div(A.x - xF, ih), kxC = hat_div(A.x - xC, ih);
+            const float kyF = hat_div(A.y - yF, ih), kyC = hat_div(A.y - yC, ih);
+            pu = kxF * kyC;
+            pv = kxC * kyF;
+            pw = kxC * kyC;
+            keep = centre || pu != 0.f || pv != 0.f || pw != 0.f;
           }
+          const uint64_t m = __ballot(keep);
+          kept[bi] = m;
+          if (keep) {
+            const int q = p2g_slot(nkept + __popcll(m & ((1ull << lane) - 1)));
+            sA[q] = make_float4(pu, pv, pw, A.z);
+            sB[q] = make_float4(A.w, B.x, B.y, B.z);
+            if (K == 2) sF1[q] = B.w;
+            if (K > 2) sC[q] = make_float4(B.w, Cc.x, Cc.y, 0.f);
+          }
+          nkept += __popcll(m);
         }
         __syncthreads();
-        const int j0 = max(lo, cs), j1 = min(hi, ce);
-        for (int j = j0; j < j1; ++j) {
-          const int q = p2g_slot(j - cs);
+        // This lane's range in staged (kept) order: kept particles before offset o.
+        auto kept_before = [&](int o) {
+          int c = 0;
+#pragma unroll
+          for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
+            const int b0 = bi * kP2GWave;
+            if (o >= b0 + kP2GWave) c += __popcll(kept[bi]);
+            else if (o > b0) c += __popcll(kept[bi] & ((1ull << (o - b0)) - 1));
+          }
+          return c;
+        };
+        const int c0 = kept_before(clampi(lo - cs, 0, nst));
+        const int c1 = kept_before(clampi(hi - cs, 0, nst));
+        for (int c = c0; c < c1; ++c) {
+          const int q = p2g_slot(c);
           const float4 RA = sA[q];
           const float4 RB = sB[q];
           const float kzF = hat_div(RA.w - zF, ih), kzC = hat_div(RA.w - zC, ih);
@@ -389,7 +422,8 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
           w0 += wx;
           w1 += wy;
           w2 += wz;
-          if (centre && j >= ownLo && j < ownHi) {
+          // the centre slab keeps every particle, so c is its offset in the chunk
+          if (centre && cs + c >= ownLo && cs + c < ownHi) {
             const float wc = RA.z * kzC;  // cell centre (x+.5, y+.5, z+.5)
             f[0] += wc * RB.w;
             if (K == 2) f[1] += wc * sF1[q];
@@ -597,33 +631,41 @@ void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
 // otherwise (sum of the six clamp-to-edge neighbours + rhs) / 6, summed in the
 // reference order x+1, x-1, y+1, y-1, z+1, z-1, rhs.
 // AIR cells are recognised from the right-hand side's reserved bit pattern.
+// All seven loads are issued before the AIR test, so a cell costs one memory
+// round trip; an AIR cell's (unused) sum is discarded. `full` only documents the
+// caller's state: AIR cells are always written (0), which is what a full sweep
+// needs and what a non-full sweep's buffer already holds.
 __device__ __forceinline__ void jacobi_cell(const Geom& G, const float* __restrict__ rhs,
                                             const float* __restrict__ pin, float* __restrict__ pout,
                                             int n, int full) {
   int x, y, z;
   cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
-  const float b = rhs[gi];
-  if (__float_as_uint(b) == kAirRhsBits) {
-    if (full) pout[gi] = 0.f;
-    return;
-  }
   const long long xp = x < G.X - 1 ? G.sx : 0, xm = x > 0 ? G.sx : 0;
   const long long yp = y < G.Y - 1 ? G.EZ : 0, ym = y > 0 ? G.EZ : 0;
   const long long zp = z < G.Z - 1 ? 1 : 0, zm = z > 0 ? 1 : 0;
+  const float b = rhs[gi];
+  const float v0 = pin[gi + xp], v1 = pin[gi - xm], v2 = pin[gi + yp];
+  const float v3 = pin[gi - ym], v4 = pin[gi + zp], v5 = pin[gi - zm];
+  // Empty asm that consumes the seven values: keeps the compiler from moving the
+  // neighbour loads behind the AIR test (a second round trip).
+  asm volatile("" ::"v"(b), "v"(v0), "v"(v1), "v"(v2), "v"(v3), "v"(v4), "v"(v5));
   float s = 0.f;
-  s += pin[gi + xp];
-  s += pin[gi - xm];
-  s += pin[gi + yp];
-  s += pin[gi - ym];
-  s += pin[gi + zp];
-  s += pin[gi - zm];
+  s += v0;
+  s += v1;
+  s += v2;
+  s += v3;
+  s += v4;
+  s += v5;
   s += b;
-  pout[gi] = div_exact(s, 1.0 / 6.0);
+  // AIR cells get 0; without `full` the buffer already holds 0 there, so storing
+  // unconditionally keeps every load on the path of the store.
+  const bool air = __float_as_uint(b) == kAirRhsBits;
+  pout[gi] = air ? 0.f : div_exact(s, 1.0 / 6.0);
 }
 
 __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
-                         float* __restrict__ pout, const uint8_t* __restrict__ active, int full) {
+                         float* __restrict__ pout, const uint32_t* __restrict__ active, int full) {
   if (!full && !active[xcd_block()]) return;
   const int n = gtid();
   if (n >= G.ncells_hash) return;
@@ -632,99 +674,6 @@ __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __r
 void launch_jacobi(const Geom& G, Grid gr, int pin, int full, hipStream_t s) {
   hipLaunchKernelGGL(k_jacobi, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr.rhs,
                      gr.p[pin], gr.p[pin ^ 1], gr.active, full);
-}
-
-// ------------------------------------------------------------------ multi-sweep kernels
-// Active launch blocks (any FLUID cell) as an ascending list, so persistent sweeps
-// spread the work evenly. One block of kCompact threads, each owning a contiguous
-// run of flags.
-constexpr int kCompact = 1024;
-__global__ __launch_bounds__(kCompact) void k_compact_active(const uint8_t* __restrict__ active,
-                                                            int nb, int* __restrict__ list,
-                                                            int* __restrict__ count) {
-  __shared__ int sh[kCompact];
-  const int t = threadIdx.x;
-  const int per = (nb + kCompact - 1) / kCompact;
-  const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
-  int c = 0;
-  for (int b = b0; b < b1; ++b) c += active[b] ? 1 : 0;
-  sh[t] = c;
-  __syncthreads();
-  for (int o = 1; o < kCompact; o <<= 1) {  // inclusive Hillis-Steele scan
-    const int v = t >= o ? sh[t - o] : 0;
-    __syncthreads();
-    sh[t] += v;
-    __syncthreads();
-  }
-  int w = sh[t] - c;
-  for (int b = b0; b < b1; ++b)
-    if (active[b]) list[w++] = b;
-  if (t == kCompact - 1) count[0] = sh[t];
-}
-void launch_compact_active(const Geom& G, Grid gr, Sweeps S, hipStream_t s) {
-  hipLaunchKernelGGL(k_compact_active, dim3(1), dim3(kCompact), 0, s, gr.active,
-                     nblocks(G.ncells_hash), S.list, S.count);
-}
-
-// Grid-wide barrier for a launch whose blocks are all resident at once (the grid
-// never exceeds the occupancy-derived capacity, and one stream runs one kernel at
-// a time). One thread per block publishes its block's writes (agent-scope release),
-// counts itself in and polls until `target` blocks have arrived, then acquires.
-// The poll is bounded: on time-out the launch records an error and carries on, so
-// every wave ends; the host reports the error and the results are void.
-// A barrier normally completes in microseconds; the limit (~10^5 polls) is three
-// orders of magnitude above that. Once any block has timed out, the others stop
-// waiting as soon as they see the error flag.
-constexpr int kSpinLimit = 1 << 16;
-__device__ __forceinline__ void grid_sync(unsigned* bar, unsigned target, int* err) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    atomicAdd(bar, 1u);
-    int spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > kSpinLimit) {
-        atomicOr(err, 1);
-        break;
-      }
-      if ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-    }
-    __threadfence();
-  }
-  __syncthreads();
-}
-// After its last barrier every block counts itself out; the last one resets both
-// counters for the next launch (nobody polls them any more by then).
-__device__ __forceinline__ void grid_done(unsigned* bar, unsigned* done) {
-  if (threadIdx.x == 0) {
-    if (atomicAdd(done, 1u) == gridDim.x - 1) {
-      atomicExch(bar, 0u);
-      atomicExch(done, 0u);
-    }
-  }
-}
-
-// `sweeps` Jacobi sweeps in one launch over the active blocks only (both buffers
-// must already hold 0 on AIR cells), one grid barrier between sweeps. Per-cell
-// arithmetic is jacobi_cell's, so results equal `sweeps` k_jacobi launches.
-__global__ __launch_bounds__(kBlock) void k_jacobi_multi(Geom G, const float* __restrict__ rhs,
-                                                         float* pa, float* pb, Sweeps S, int sweeps) {
-  const int na = S.count[0];
-  for (int it = 0; it < sweeps; ++it) {
-    const float* pin = (it & 1) ? pb : pa;
-    float* pout = (it & 1) ? pa : pb;
-    for (int t = blockIdx.x; t < na; t += gridDim.x) {
-      const int n = S.list[t] * kBlock + (int)threadIdx.x;
-      if (n < G.ncells_hash) jacobi_cell(G, rhs, pin, pout, n, 0);
-    }
-    if (it + 1 < sweeps) grid_sync(S.bar, (unsigned)(it + 1) * gridDim.x, S.err);
-  }
-  grid_done(S.bar, S.bar + 1);
-}
-void launch_jacobi_multi(const Geom& G, Grid gr, int pin, int sweeps, Sweeps S, hipStream_t s) {
-  hipLaunchKernelGGL(k_jacobi_multi, dim3(S.grid), dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin],
-                     gr.p[pin ^ 1], S, sweeps);
 }
 
 // diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
@@ -870,27 +819,41 @@ __device__ __forceinline__ void diffusion_cell(const Geom& G, const Grid& gr, in
   int x, y, z;
   cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
-  if (gr.content[gi] == AIR) {
-    if (full) {
-#pragma unroll
-      for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = gr.nvf[fin][a][gi];
-    }
-    return;
-  }
   const long long nb[6] = {gi + (x < G.X - 1 ? G.sx : 0), gi - (x > 0 ? G.sx : 0),
                            gi + (y < G.Y - 1 ? (long long)G.EZ : 0),
                            gi - (y > 0 ? (long long)G.EZ : 0), gi + (z < G.Z - 1 ? 1 : 0),
                            gi - (z > 0 ? 1 : 0)};
+  // Every load (content, the cell's and its six neighbours' channels) is issued
+  // before the AIR test: one memory round trip per cell.
+  const uint8_t c = gr.content[gi];
+  float v[7][K];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int a = 0; a < K; ++a) v[k][a] = gr.nvf[fin][a][nb[k]];
+#pragma unroll
+  for (int a = 0; a < K; ++a) v[6][a] = gr.nvf[fin][a][gi];
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+#pragma unroll
+    for (int a = 0; a < K; ++a) asm volatile("" ::"v"(v[k][a]));
+  if (c == AIR) {
+    if (full) {
+#pragma unroll
+      for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = v[6][a];
+    }
+    return;
+  }
   float r[K];
 #pragma unroll
   for (int a = 0; a < K; ++a) r[a] = 0.f;
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
 #pragma unroll
-    for (int a = 0; a < K; ++a) r[a] += gr.nvf[fin][a][nb[k]] * lambda;
+    for (int a = 0; a < K; ++a) r[a] += v[k][a] * lambda;
   }
 #pragma unroll
-  for (int a = 0; a < K; ++a) r[a] += gr.nvf[fin][a][gi];
+  for (int a = 0; a < K; ++a) r[a] += v[6][a];
 #pragma unroll
   for (int a = 0; a < K; ++a) r[a] = div_exact(r[a], inv_cc);  // r / (1 + 2 lambda)
   float sum = r[0];
@@ -916,47 +879,6 @@ void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, doub
     case 3: hipLaunchKernelGGL(k_diffusion<3>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, full); break;
     default: hipLaunchKernelGGL(k_diffusion<4>, dim3(nb), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, full); break;
   }
-}
-
-// `sweeps` diffusion sweeps in one launch over the active blocks (both buffers must
-// already agree on AIR cells), one grid barrier between sweeps.
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_diffusion_multi(Geom G, Grid gr, int fin, float lambda,
-                                                            double inv_cc, Sweeps S, int sweeps) {
-  const int na = S.count[0];
-  for (int it = 0; it < sweeps; ++it) {
-    const int f = fin ^ (it & 1);
-    for (int t = blockIdx.x; t < na; t += gridDim.x) {
-      const int n = S.list[t] * kBlock + (int)threadIdx.x;
-      if (n < G.ncells_hash) diffusion_cell<K>(G, gr, f, lambda, inv_cc, n, 0);
-    }
-    if (it + 1 < sweeps) grid_sync(S.bar, (unsigned)(it + 1) * gridDim.x, S.err);
-  }
-  grid_done(S.bar, S.bar + 1);
-}
-void launch_diffusion_multi(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc,
-                            int sweeps, Sweeps S, hipStream_t s) {
-  switch (K) {
-    case 1: hipLaunchKernelGGL(k_diffusion_multi<1>, dim3(S.grid), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, S, sweeps); break;
-    case 2: hipLaunchKernelGGL(k_diffusion_multi<2>, dim3(S.grid), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, S, sweeps); break;
-    case 3: hipLaunchKernelGGL(k_diffusion_multi<3>, dim3(S.grid), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, S, sweeps); break;
-    default: hipLaunchKernelGGL(k_diffusion_multi<4>, dim3(S.grid), dim3(kBlock), 0, s, G, gr, fin, lambda, inv_cc, S, sweeps); break;
-  }
-}
-// Largest grid of k_jacobi_multi / k_diffusion_multi<K> whose blocks fit on the
-// device at once.
-int multi_sweep_capacity(int K) {
-  int dev = 0, cus = 0, a = 0, b = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, (const void*)k_jacobi_multi, kBlock, 0) != hipSuccess)
-    return 0;
-  const void* fd = K == 1 ? (const void*)k_diffusion_multi<1>
-                 : K == 2 ? (const void*)k_diffusion_multi<2>
-                 : K == 3 ? (const void*)k_diffusion_multi<3>
-                          : (const void*)k_diffusion_multi<4>;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fd, kBlock, 0) != hipSuccess) return 0;
-  return cus * std::min(a, b);
 }
 
 // ------------------------------------------------------------------ G2P and advection

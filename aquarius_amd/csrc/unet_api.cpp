@@ -81,6 +81,15 @@ struct aq_unet {
   double *lpart = nullptr, *lout = nullptr, *sig = nullptr;
   std::vector<void*> act_allocs;
 
+  // warm-start guess: dense per-level fields (level 0 = the FLIP grid), sized for
+  // the largest grid seen
+  long long guess_cap = 0;
+  std::vector<float*> gb, gm, gp;
+  float *res = nullptr, *corr = nullptr;  // residual and correction, FLIP grid layout
+  long long res_cap = 0;
+  std::vector<void*> guess_allocs, res_allocs;
+  double* alpha = nullptr;
+
   // dataset
   int ds_cap = 0, ds_n = 0, ds_D = 0, ds_H = 0, ds_W = 0;
   __bf16* ds_x = nullptr;
@@ -88,6 +97,8 @@ struct aq_unet {
 
   ~aq_unet() {
     if (stream) hipStreamSynchronize(stream);
+    for (void* p : guess_allocs) hipFree(p);
+    for (void* p : res_allocs) hipFree(p);
     for (void* p : act_allocs) hipFree(p);
     for (void* p : allocs) hipFree(p);
     if (stream) hipStreamDestroy(stream);
@@ -97,6 +108,37 @@ struct aq_unet {
     UHIP(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
     list.push_back(*p);
     UHIP(hipMemsetAsync(*p, 0, std::max<size_t>(n, 1) * sizeof(T), stream));
+    return AQ_OK;
+  }
+
+  // Dense guess fields for a grid of V cells and `levels` restrictions.
+  aq_status ensure_guess(long long V, int levels) {
+    if (V <= guess_cap && (int)gb.size() > levels) return AQ_OK;
+    UHIP(hipStreamSynchronize(stream));
+    for (void* p : guess_allocs) hipFree(p);
+    guess_allocs.clear();
+    gb.assign(levels + 1, nullptr);
+    gm.assign(levels + 1, nullptr);
+    gp.assign(levels + 1, nullptr);
+    for (int l = 0; l <= levels; ++l) {
+      const long long Vl = (V >> (3 * l)) + 1;
+      UTRY(alloc(&gb[l], Vl, guess_allocs));
+      UTRY(alloc(&gm[l], Vl, guess_allocs));
+      UTRY(alloc(&gp[l], Vl, guess_allocs));
+    }
+    guess_cap = V;
+    return AQ_OK;
+  }
+
+  // Residual and correction fields in the FLIP grid's padded layout (ncell floats).
+  aq_status ensure_res(long long ncell) {
+    if (ncell <= res_cap) return AQ_OK;
+    UHIP(hipStreamSynchronize(stream));
+    for (void* p : res_allocs) hipFree(p);
+    res_allocs.clear();
+    UTRY(alloc(&res, ncell, res_allocs));
+    UTRY(alloc(&corr, ncell, res_allocs));
+    res_cap = ncell;
     return AQ_OK;
   }
 
@@ -221,6 +263,20 @@ void nn_target(const aqf::Geom& G, const uint8_t* content, const float* p, const
 void nn_output(const aqf::Geom& G, const uint8_t* content, const float* y, const double* sig,
                float scale, float* p, hipStream_t s);
 void host_input(long long V, const float* x2, __bf16* x0, hipStream_t s);
+void nn_guess_gather(const aqf::Geom& G, const uint8_t* content, const float* rhs, float* b, float* m,
+                     hipStream_t s);
+void nn_restrict(int Xc, int Yc, int Zc, const float* bf, const float* mf, float* bc, float* mc,
+                 hipStream_t s);
+void nn_input_dense(long long V, const float* b, const float* m, double* part, int nparts, double* sig,
+                    __bf16* x0, hipStream_t s);
+void nn_output_dense(long long V, const float* y, const float* m, const double* sig, float scale, float* p,
+                     hipStream_t s);
+void nn_prolong(int Xc, int Yc, int Zc, const float* pc, float* pf, hipStream_t s);
+void nn_scatter_scaled(const aqf::Geom& G, const uint8_t* content, const float* rhs, const float* pf,
+                       float* p, double* part, int nparts, double* alpha, hipStream_t s);
+void nn_residual(const aqf::Geom& G, const uint8_t* content, const float* rhs, const float* p, float* r,
+                 hipStream_t s);
+void nn_axpy(const aqf::Geom& G, const uint8_t* content, const float* d, float* p, hipStream_t s);
 }  // namespace aqu
 
 namespace {
@@ -228,6 +284,36 @@ namespace {
 float out_scale(const aq_unet* net, int D) {
   const double r = (double)D / 64.0;
   return (float)(net->cfg.target_scale * r * r);
+}
+
+// dst = alpha * net(src) on FLUID cells (0 elsewhere) for the FLIP grid of fv, where
+// src is a right-hand side in the grid layout (the pressure right-hand side, or a
+// residual) and alpha = <d, src> / <d, A d> for the network's output d. Grids larger
+// than cfg.infer_extent are restricted 2x per level for the network and its output
+// prolonged back. Runs on the network's stream; the caller synchronises.
+aq_status nn_infer(const aqf::FlipView& fv, aq_unet* net, const float* src, float* dst) {
+  const aqf::Geom& G = fv.G;
+  int levels = 0;  // halve while larger than the network's resolution and still divisible
+  while (std::max(G.X, std::max(G.Y, G.Z)) >> levels > net->cfg.infer_extent &&
+         (G.X >> levels) % 8 == 0 && (G.Y >> levels) % 8 == 0 && (G.Z >> levels) % 8 == 0)
+    ++levels;
+  const int X = G.X >> levels, Y = G.Y >> levels, Z = G.Z >> levels;
+  if (X % 4 || Y % 4 || Z % 4) return ufail(AQ_EINVAL, "grid extents must be multiples of 4");
+  const long long V0 = (long long)G.X * G.Y * G.Z, V = (long long)X * Y * Z;
+  UTRY(net->ensure(V));
+  UTRY(net->ensure_guess(V0, levels));
+  hipStream_t s = net->stream;
+  nn_guess_gather(G, fv.gr->content, src, net->gb[0], net->gm[0], s);
+  for (int l = 1; l <= levels; ++l)
+    nn_restrict(G.X >> l, G.Y >> l, G.Z >> l, net->gb[l - 1], net->gm[l - 1], net->gb[l], net->gm[l], s);
+  nn_input_dense(V, net->gb[levels], net->gm[levels], net->lpart, kRedBlocks, net->sig, net->x0, s);
+  net->forward(Vol{1, X, Y, Z});
+  nn_output_dense(V, net->y, net->gm[levels], net->sig, out_scale(net, X), net->gp[levels], s);
+  for (int l = levels; l >= 1; --l)
+    nn_prolong(G.X >> l, G.Y >> l, G.Z >> l, net->gp[l], net->gp[l - 1], s);
+  nn_scatter_scaled(G, fv.gr->content, src, net->gp[0], dst, net->lpart, kRedBlocks, net->alpha, s);
+  UHIP(hipGetLastError());
+  return AQ_OK;
 }
 
 aq_status upload_input(aq_unet* net, const float* x, long long V) {
@@ -254,6 +340,7 @@ aq_status aq_unet_config_default(aq_unet_config* c) {
   c->beta2 = 0.999f;
   c->eps = 1e-8f;
   c->target_scale = 64.f;
+  c->infer_extent = 64;
   return AQ_OK;
 }
 
@@ -262,6 +349,7 @@ aq_status aq_unet_create(const aq_unet_config* cfg, aq_unet** out) {
   if (cfg->channels[0] != 16 || cfg->channels[1] != 32 || cfg->channels[2] != 64)
     return ufail(AQ_EINVAL, "the U-Net kernels are built for 16/32/64 channels");
   if (!(cfg->lr > 0) || !(cfg->target_scale > 0)) return ufail(AQ_EINVAL, "lr and target_scale must be > 0");
+  if (cfg->infer_extent < 4) return ufail(AQ_EINVAL, "infer_extent must be >= 4");
   aq_unet* n = new aq_unet;
   n->cfg = *cfg;
   auto bail = [&](aq_status s) {
@@ -297,6 +385,7 @@ aq_status aq_unet_create(const aq_unet_config* cfg, aq_unet** out) {
   if (st == AQ_OK) st = n->alloc(&n->wpart, maxw * kNSplit, n->allocs);
   if (st == AQ_OK) st = n->alloc(&n->bpart, (size_t)kRedBlocks * 64, n->allocs);
   if (st == AQ_OK) st = n->alloc(&n->lpart, (size_t)kRedBlocks * 2, n->allocs);
+  if (st == AQ_OK) st = n->alloc(&n->alpha, 1, n->allocs);
   if (st == AQ_OK) st = n->alloc(&n->lout, 4, n->allocs);
   if (st == AQ_OK) st = n->alloc(&n->sig, 4, n->allocs);
   if (st != AQ_OK) return bail(st);
@@ -480,16 +569,23 @@ aq_status aq_unet_eval(aq_unet* net, int32_t first, int32_t count, double* loss)
 aq_status aq_flip_nn_guess(aq_flip* flip, aq_unet* net) {
   if (!net || !flip) return ufail(AQ_EINVAL, "null argument");
   aqf::FlipView fv = aqf::flip_view(flip);
-  if (fv.G.X % 4 || fv.G.Y % 4 || fv.G.Z % 4) return ufail(AQ_EINVAL, "grid extents must be multiples of 4");
-  const long long V = (long long)fv.G.X * fv.G.Y * fv.G.Z;
-  UTRY(net->ensure(V));
   // The flip stream's work (divergence) must finish before the net reads it, and the
   // guess must land before the flip stream continues: both streams meet here.
   UHIP(hipStreamSynchronize(fv.stream));
-  nn_sigma(fv.G, fv.gr->content, fv.gr->rhs, net->lpart, kRedBlocks, net->sig, net->stream);
-  nn_input(fv.G, fv.gr->content, fv.gr->rhs, net->sig, net->x0, net->mask, net->stream);
-  net->forward(Vol{1, fv.G.X, fv.G.Y, fv.G.Z});
-  nn_output(fv.G, fv.gr->content, net->y, net->sig, out_scale(net, fv.G.X), fv.gr->p[*fv.pb], net->stream);
+  UTRY(nn_infer(fv, net, fv.gr->rhs, fv.gr->p[*fv.pb]));
+  UHIP(hipStreamSynchronize(net->stream));
+  return AQ_OK;
+}
+
+aq_status aq_flip_nn_correct(aq_flip* flip, aq_unet* net) {
+  if (!net || !flip) return ufail(AQ_EINVAL, "null argument");
+  aqf::FlipView fv = aqf::flip_view(flip);
+  UTRY(net->ensure_res(fv.G.ncell));
+  UHIP(hipStreamSynchronize(fv.stream));
+  float* p = fv.gr->p[*fv.pb];
+  nn_residual(fv.G, fv.gr->content, fv.gr->rhs, p, net->res, net->stream);
+  UTRY(nn_infer(fv, net, net->res, net->corr));
+  nn_axpy(fv.G, fv.gr->content, net->corr, p, net->stream);
   UHIP(hipGetLastError());
   UHIP(hipStreamSynchronize(net->stream));
   return AQ_OK;

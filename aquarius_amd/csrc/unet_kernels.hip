@@ -50,9 +50,16 @@ __device__ __forceinline__ long long vox_id(const Vol& v, int b, int d, int h, i
 
 // ------------------------------------------------------------------ conv 3x3x3, implicit GEMM
 // Y[v, n] = sum_{tap, c} X[v + off(tap), c] * Wk[n, tap*CIN + c] (+ bias, ReLU),
-// zero padding outside the volume. M = voxels (16 per wave), N = COUT, K = 27*CIN
-// in slices of 32. With flipped, transposed weights the same kernel computes the
+// zero padding outside the volume. M = voxels, N = COUT, K = 27*CIN in slices of
+// 32. A wave owns kConvMF fragments of 16 consecutive voxels, so every weight
+// fragment it fetches feeds kConvMF MFMAs and the fragments' input gathers are in
+// flight together. When the layer's weights fit (<= 64 KiB) the block stages them
+// in LDS once, rows padded by 16 bytes so the 16 rows a wave reads fall in
+// distinct banks. With flipped, transposed weights the same kernel computes the
 // input gradient. OUT_F32 writes FP32 (optionally accumulating), else bf16.
+constexpr int kConvMF = 4;
+constexpr int kConvVox = 4 * kConvMF * 16;  // voxels per 256-thread block
+
 template <int CIN, int COUT, bool RELU, bool OUT_F32, bool ACCUM>
 __global__ __launch_bounds__(256) void k_conv3(Vol v, const __bf16* __restrict__ X, int xs, int xo,
                                                const __bf16* __restrict__ Wk,
@@ -60,60 +67,90 @@ __global__ __launch_bounds__(256) void k_conv3(Vol v, const __bf16* __restrict__
                                                int ys, int yo) {
   constexpr int K = 27 * CIN;
   constexpr int NT = COUT / 16;
+  constexpr int MF = kConvMF;
+  constexpr int RS = K + 8;  // LDS row stride in bf16
+  constexpr bool WLDS = COUT * RS * 2 <= 64 * 1024;
+  __shared__ __align__(16) __bf16 sW[WLDS ? COUT * RS : 8];
+  if (WLDS) {
+    for (int e = threadIdx.x; e < COUT * (K / 8); e += 256) {
+      const int n = e / (K / 8), j = e - n * (K / 8);
+      *(bf16x8*)&sW[n * RS + j * 8] = *(const bf16x8*)(Wk + (long long)n * K + j * 8);
+    }
+    __syncthreads();
+  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long V = (long long)v.B * v.D * v.H * v.W;
-  const long long vbase = ((long long)xcd_block() * 4 + wave) * 16;
-  if (vbase >= V) return;
+  const long long wbase = ((long long)xcd_block() * 4 + wave) * (MF * 16);
+  if (wbase >= V) return;
   const int r = lane & 15, q = lane >> 4;
-  const long long vid = vbase + r;
-  const bool vok = vid < V;
-  const VoxelPos p = vox_of(v, vok ? vid : 0);
-  f32x4 acc[NT];
+  VoxelPos p[MF];
+  bool vok[MF];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int f = 0; f < MF; ++f) {
+    const long long vid = wbase + f * 16 + r;
+    vok[f] = vid < V;
+    p[f] = vox_of(v, vok[f] ? vid : 0);
+  }
+  f32x4 acc[MF][NT];
+#pragma unroll
+  for (int f = 0; f < MF; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
   const bf16x8 zero8 = {};
   for (int k0 = 0; k0 < K; k0 += 32) {
     const int k = k0 + q * 8;
-    bf16x8 a = zero8;
-    if (k < K && vok) {
-      const int tap = k / CIN, c = k - tap * CIN;
-      const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
-      const int zd = p.d + dd, zh = p.h + dh, zw = p.w + dw;
-      if (zd >= 0 && zd < v.D && zh >= 0 && zh < v.H && zw >= 0 && zw < v.W)
-        a = *(const bf16x8*)(X + vox_id(v, p.b, zd, zh, zw) * xs + xo + c);
+    const bool kok = k < K;
+    const int tap = k / CIN, c = k - tap * CIN;
+    const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
+    bf16x8 a[MF];
+#pragma unroll
+    for (int f = 0; f < MF; ++f) {
+      a[f] = zero8;
+      const int zd = p[f].d + dd, zh = p[f].h + dh, zw = p[f].w + dw;
+      if (kok && vok[f] && zd >= 0 && zd < v.D && zh >= 0 && zh < v.H && zw >= 0 && zw < v.W)
+        a[f] = *(const bf16x8*)(X + vox_id(v, p[f].b, zd, zh, zw) * xs + xo + c);
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       bf16x8 bw = zero8;
-      if (k < K) bw = *(const bf16x8*)(Wk + (long long)(t * 16 + r) * K + k);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
+      if (kok) {
+        if (WLDS)
+          bw = *(const bf16x8*)&sW[(t * 16 + r) * RS + k];
+        else
+          bw = *(const bf16x8*)(Wk + (long long)(t * 16 + r) * K + k);
+      }
+#pragma unroll
+      for (int f = 0; f < MF; ++f)
+        acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[f], bw, acc[f][t], 0, 0, 0);
     }
   }
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int n = t * 16 + r;
+  for (int f = 0; f < MF; ++f)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const long long o = vbase + q * 4 + i;
-      if (o >= V) continue;
-      float val = acc[t][i];
-      if (bias) val += bias[n];
-      if (RELU) val = fmaxf(val, 0.f);
-      if (OUT_F32) {
-        float* Y = (float*)Yv + o * ys + yo + n;
-        *Y = ACCUM ? *Y + val : val;
-      } else {
-        ((__bf16*)Yv)[o * ys + yo + n] = f2bf(val);
+    for (int t = 0; t < NT; ++t) {
+      const int n = t * 16 + r;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const long long o = wbase + f * 16 + q * 4 + i;
+        if (o >= V) continue;
+        float val = acc[f][t][i];
+        if (bias) val += bias[n];
+        if (RELU) val = fmaxf(val, 0.f);
+        if (OUT_F32) {
+          float* Y = (float*)Yv + o * ys + yo + n;
+          *Y = ACCUM ? *Y + val : val;
+        } else {
+          ((__bf16*)Yv)[o * ys + yo + n] = f2bf(val);
+        }
       }
     }
-  }
 }
 
 template <int CIN, int COUT>
 static void conv3_fwd_t(const Vol& v, const __bf16* X, int xs, int xo, const __bf16* Wk,
                         const float* bias, __bf16* Y, int ys, int yo, bool relu, hipStream_t s) {
   const long long V = (long long)v.B * v.D * v.H * v.W;
-  const int nb = (int)((V + 63) / 64);
+  const int nb = (int)((V + kConvVox - 1) / kConvVox);
   if (relu)
     hipLaunchKernelGGL((k_conv3<CIN, COUT, true, false, false>), dim3(nb), dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
   else
@@ -124,7 +161,7 @@ static void conv3_dgrad_t(const Vol& v, const __bf16* dY, const __bf16* Wt, floa
                           hipStream_t s) {
   // transposed problem: input dYpre has CIN (= the layer's COUT) channels, output COUT (= its CIN)
   const long long V = (long long)v.B * v.D * v.H * v.W;
-  const int nb = (int)((V + 63) / 64);
+  const int nb = (int)((V + kConvVox - 1) / kConvVox);
   hipLaunchKernelGGL((k_conv3<CIN, COUT, false, true, false>), dim3(nb), dim3(256), 0, s, v, dY, CIN, 0, Wt, (const float*)nullptr, (void*)dX, dxs, dxo);
 }
 
@@ -674,6 +711,257 @@ void nn_output(const aqf::Geom& G, const uint8_t* content, const float* y, const
                float* p, hipStream_t s) {
   hipLaunchKernelGGL(k_nn_output, dim3((G.ncells_hash + 255) / 256), dim3(256), 0, s, G, content, y, sig,
                      scale, p);
+}
+
+// ------------------------------------------------------------------ warm-start guess
+// The guess runs on dense fields in hash order (x-major, z-fastest). On grids
+// larger than the training resolution the right-hand side is restricted 2x per
+// level (sum of the eight children; a coarse cell is FLUID when at least four of
+// its children are), the network runs on the coarsest level, and its pressure is
+// prolonged back with cell-centred trilinear weights (3/4, 1/4 per axis). The
+// fine guess p is finally scaled by alpha = <p, b> / <p, A p> (A = the pressure
+// operator of the FP64 residual, symmetric positive definite), the alpha that
+// minimises the energy-norm error ||p* - alpha p||_A with A p* = b: whatever the
+// network's scale error, the scaled guess is never further from the solution in
+// that norm than a zero guess. (Minimising the residual norm instead would weigh
+// the guess's high-frequency error, which the first sweeps remove anyway, and
+// shrink its useful smooth part.)
+__global__ void k_nn_gather(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ rhs,
+                            float* __restrict__ b, float* __restrict__ m) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell3(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  const bool fl = content[gi] == aqf::FLUID;
+  b[n] = fl ? rhs[gi] : 0.f;
+  m[n] = fl ? 1.f : 0.f;
+}
+__global__ void k_nn_restrict(int Xc, int Yc, int Zc, const float* __restrict__ bf, const float* __restrict__ mf,
+                              float* __restrict__ bc, float* __restrict__ mc) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long long)Xc * Yc * Zc) return;
+  const int z = (int)(n % Zc);
+  const long long t = n / Zc;
+  const int y = (int)(t % Yc), x = (int)(t / Yc);
+  const int Yf = 2 * Yc, Zf = 2 * Zc;
+  float sb = 0.f, sm = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const long long f = ((long long)(2 * x + (i >> 2)) * Yf + (2 * y + ((i >> 1) & 1))) * Zf + 2 * z + (i & 1);
+    sb += bf[f];
+    sm += mf[f];
+  }
+  const bool fl = sm >= 4.f;
+  bc[n] = fl ? sb : 0.f;
+  mc[n] = fl ? 1.f : 0.f;
+}
+__global__ void k_nn_sigma_dense(long long V, const float* __restrict__ b, const float* __restrict__ m,
+                                 double* __restrict__ part) {
+  __shared__ double s1[256], s2[256];
+  double a = 0, c = 0;
+  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < V;
+       n += (long long)gridDim.x * blockDim.x) {
+    if (m[n] == 0.f) continue;
+    const double v = (double)b[n];
+    a += v * v;
+    c += 1.0;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+__global__ void k_nn_input_dense(long long V, const float* __restrict__ b, const float* __restrict__ m,
+                                 const double* __restrict__ sig, __bf16* __restrict__ x0) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= V) return;
+  __bf16* o = x0 + n * 8;
+  o[0] = f2bf((float)((double)b[n] * sig[1]));
+  o[1] = f2bf(m[n]);
+#pragma unroll
+  for (int c = 2; c < 8; ++c) o[c] = f2bf(0.f);
+}
+__global__ void k_nn_output_dense(long long V, const float* __restrict__ y, const float* __restrict__ m,
+                                  const double* __restrict__ sig, float scale, float* __restrict__ p) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= V) return;
+  p[n] = m[n] != 0.f ? (float)((double)y[n] * sig[0] * (double)scale) : 0.f;
+}
+__global__ void k_nn_prolong(int Xc, int Yc, int Zc, const float* __restrict__ pc, float* __restrict__ pf) {
+  const int Xf = 2 * Xc, Yf = 2 * Yc, Zf = 2 * Zc;
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long long)Xf * Yf * Zf) return;
+  const int z = (int)(n % Zf);
+  const long long t = n / Zf;
+  const int y = (int)(t % Yf), x = (int)(t / Yf);
+  // parent and the neighbouring parent on the child's side, weights 3/4 and 1/4
+  const int px = x >> 1, py = y >> 1, pz = z >> 1;
+  const int qx = min(max(px + ((x & 1) ? 1 : -1), 0), Xc - 1);
+  const int qy = min(max(py + ((y & 1) ? 1 : -1), 0), Yc - 1);
+  const int qz = min(max(pz + ((z & 1) ? 1 : -1), 0), Zc - 1);
+  const int ix[2] = {px, qx}, iy[2] = {py, qy}, iz[2] = {pz, qz};
+  const float w[2] = {0.75f, 0.25f};
+  float s = 0.f;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        s += w[a] * w[b] * w[c] * pc[((long long)ix[a] * Yc + iy[b]) * Zc + iz[c]];
+  pf[n] = s;
+}
+__global__ void k_nn_scatter(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ pf,
+                             float* __restrict__ p) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell3(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  p[gi] = content[gi] == aqf::FLUID ? pf[n] : 0.f;
+}
+// <p, b> and <p, A p> over FLUID cells, A as in the FP64 residual.
+__global__ void k_nn_alpha_part(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ rhs,
+                                const float* __restrict__ p, double* __restrict__ part) {
+  __shared__ double s1[256], s2[256];
+  double qb = 0, qq = 0;
+  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash;
+       n += (long long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    cell3(G, n, x, y, z);
+    const long long gi = G.g(x, y, z);
+    if (content[gi] != aqf::FLUID) continue;
+    const double pc = (double)p[gi];
+    double ap = 6.0 * pc;
+    const bool in[6] = {x + 1 < G.X, x > 0, y + 1 < G.Y, y > 0, z + 1 < G.Z, z > 0};
+    const long long o[6] = {G.sx, -G.sx, (long long)G.EZ, -(long long)G.EZ, 1, -1};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (!in[k])
+        ap -= pc;
+      else if (content[gi + o[k]] == aqf::FLUID)
+        ap -= (double)p[gi + o[k]];
+    }
+    qb += pc * (double)rhs[gi];
+    qq += pc * ap;
+  }
+  s1[threadIdx.x] = qb;
+  s2[threadIdx.x] = qq;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+__global__ void k_nn_alpha_fold(const double* __restrict__ part, int n, double* __restrict__ alpha) {
+  if (threadIdx.x != 0) return;
+  double qb = 0, qq = 0;
+  for (int i = 0; i < n; ++i) {
+    qb += part[2 * i];
+    qq += part[2 * i + 1];
+  }
+  alpha[0] = (qq > 0 && isfinite(qb / qq)) ? qb / qq : 0.0;
+}
+__global__ void k_nn_alpha_apply(aqf::Geom G, const double* __restrict__ alpha, float* __restrict__ p) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell3(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  p[gi] = (float)((double)p[gi] * alpha[0]);
+}
+
+// r = b - A p over FLUID cells (FP64 arithmetic, stored FP32), 0 elsewhere.
+__global__ void k_nn_residual(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ rhs,
+                              const float* __restrict__ p, float* __restrict__ r) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell3(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  if (content[gi] != aqf::FLUID) {
+    r[gi] = 0.f;
+    return;
+  }
+  const double pc = (double)p[gi];
+  double ap = 6.0 * pc;
+  const bool in[6] = {x + 1 < G.X, x > 0, y + 1 < G.Y, y > 0, z + 1 < G.Z, z > 0};
+  const long long o[6] = {G.sx, -G.sx, (long long)G.EZ, -(long long)G.EZ, 1, -1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (!in[k])
+      ap -= pc;
+    else if (content[gi + o[k]] == aqf::FLUID)
+      ap -= (double)p[gi + o[k]];
+  }
+  r[gi] = (float)((double)rhs[gi] - ap);
+}
+// p += d on FLUID cells.
+__global__ void k_nn_axpy(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ d,
+                          float* __restrict__ p) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.ncells_hash) return;
+  int x, y, z;
+  cell3(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  if (content[gi] == aqf::FLUID) p[gi] += d[gi];
+}
+
+static inline int nblk(long long n) { return (int)((n + 255) / 256); }
+
+void nn_residual(const aqf::Geom& G, const uint8_t* content, const float* rhs, const float* p, float* r,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_residual, dim3(nblk(G.ncells_hash)), dim3(256), 0, s, G, content, rhs, p, r);
+}
+void nn_axpy(const aqf::Geom& G, const uint8_t* content, const float* d, float* p, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_axpy, dim3(nblk(G.ncells_hash)), dim3(256), 0, s, G, content, d, p);
+}
+
+void nn_guess_gather(const aqf::Geom& G, const uint8_t* content, const float* rhs, float* b, float* m,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_gather, dim3(nblk(G.ncells_hash)), dim3(256), 0, s, G, content, rhs, b, m);
+}
+void nn_restrict(int Xc, int Yc, int Zc, const float* bf, const float* mf, float* bc, float* mc, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_restrict, dim3(nblk((long long)Xc * Yc * Zc)), dim3(256), 0, s, Xc, Yc, Zc, bf, mf,
+                     bc, mc);
+}
+void nn_input_dense(long long V, const float* b, const float* m, double* part, int nparts, double* sig,
+                    __bf16* x0, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_sigma_dense, dim3(nparts), dim3(256), 0, s, V, b, m, part);
+  hipLaunchKernelGGL(k_nn_sigma_fold, dim3(1), dim3(64), 0, s, part, nparts, sig);
+  hipLaunchKernelGGL(k_nn_input_dense, dim3(nblk(V)), dim3(256), 0, s, V, b, m, sig, x0);
+}
+void nn_output_dense(long long V, const float* y, const float* m, const double* sig, float scale, float* p,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_output_dense, dim3(nblk(V)), dim3(256), 0, s, V, y, m, sig, scale, p);
+}
+void nn_prolong(int Xc, int Yc, int Zc, const float* pc, float* pf, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_prolong, dim3(nblk(8LL * Xc * Yc * Zc)), dim3(256), 0, s, Xc, Yc, Zc, pc, pf);
+}
+void nn_scatter_scaled(const aqf::Geom& G, const uint8_t* content, const float* rhs, const float* pf, float* p,
+                       double* part, int nparts, double* alpha, hipStream_t s) {
+  hipLaunchKernelGGL(k_nn_scatter, dim3(nblk(G.ncells_hash)), dim3(256), 0, s, G, content, pf, p);
+  hipLaunchKernelGGL(k_nn_alpha_part, dim3(nparts), dim3(256), 0, s, G, content, rhs, p, part);
+  hipLaunchKernelGGL(k_nn_alpha_fold, dim3(1), dim3(64), 0, s, part, nparts, alpha);
+  hipLaunchKernelGGL(k_nn_alpha_apply, dim3(nblk(G.ncells_hash)), dim3(256), 0, s, G, alpha, p);
 }
 
 // Host tensor (V x 2 floats) -> network input (V x 8 bf16).

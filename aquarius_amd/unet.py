@@ -65,6 +65,9 @@ class UNet:
 
     def load_file(self, path) -> None:
         with np.load(path, allow_pickle=False) as z:
+            if "target_scale" in z and float(z["target_scale"]) != float(self.cfg.target_scale):
+                raise ValueError(f"{path}: weights were trained with target_scale "
+                                 f"{float(z['target_scale'])}, this network uses {self.cfg.target_scale}")
             self.set_params(z["params"])
 
     def forward(self, x: np.ndarray) -> np.ndarray:
@@ -112,10 +115,15 @@ def nn_guess(flip, net: UNet) -> None:
     check(load().aq_flip_nn_guess(flip.h, net.h))
 
 
+def nn_correct(flip, net: UNet) -> None:
+    check(load().aq_flip_nn_correct(flip.h, net.h))
+
+
 def attach_solver(flip, mode: str, tol: float = 1e-4, max_iters: int = 100000,
-                  check_every: int = 50, stall_checks: int = 20, net: UNet | None = None) -> None:
+                  check_every: int = 50, stall_checks: int = 20, net: UNet | None = None,
+                  nn_refresh: int = 0) -> None:
     modes = {"fixed": AQ_SOLVE_FIXED, "tol_zero": AQ_SOLVE_TOL_ZERO, "tol_nn": AQ_SOLVE_TOL_NN}
-    s = Solver(modes[mode], tol, max_iters, check_every, stall_checks)
+    s = Solver(modes[mode], tol, max_iters, check_every, stall_checks, nn_refresh)
     check(load().aq_flip_set_solver(flip.h, C.byref(s), net.h if net is not None else None))
 
 

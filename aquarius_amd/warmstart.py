@@ -84,35 +84,64 @@ def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int =
                 mean_sweeps=sweeps / max(1, net.dataset_size))
 
 
-def compare_on(h: FlipHandle, net: UNet, tol: float = 1e-4, cap: int = 200000, check_every: int = 10):
-    """Sweeps to `tol` for the handle's current system: from zero and from the U-Net guess."""
+def solve_with_corrections(h: FlipHandle, net: UNet, tol: float, cap: int, refresh: int,
+                           check_every: int = 10):
+    """From the U-Net guess: Jacobi in chunks of `refresh` sweeps, the network
+    correcting the iterate from its residual between chunks. Returns (sweeps,
+    residual, converged, corrections)."""
+    from .unet import nn_correct, nn_guess
+    nn_guess(h, net)
+    total, corrections = 0, 0
+    while True:
+        it, rel, ok = h.solve_pressure(tol, min(refresh, cap - total), check_every)
+        total += it
+        if ok or total >= cap:
+            return total, rel, ok, corrections
+        nn_correct(h, net)
+        corrections += 1
+
+
+def compare_on(h: FlipHandle, net: UNet, tol: float = 1e-4, cap: int = 200000, check_every: int = 10,
+               refresh: int | None = None):
+    """Sweeps to `tol` for the handle's current system: from zero, from the U-Net
+    guess, and (with `refresh`) from the guess with a network correction every
+    `refresh` sweeps."""
     from .unet import nn_guess
     it0, r0, ok0 = converge(h, tol, cap, check_every)
     nn_guess(h, net)
     g = h.residual()
     it1, r1, ok1 = h.solve_pressure(tol, cap, check_every)
-    return dict(plain_iters=it0, plain_residual=r0, plain_ok=ok0, nn_guess_residual=g,
-                nn_iters=it1, nn_residual=r1, nn_ok=ok1)
+    out = dict(plain_iters=it0, plain_residual=r0, plain_ok=ok0, nn_guess_residual=g,
+               nn_iters=it1, nn_residual=r1, nn_ok=ok1)
+    if refresh:
+        it2, r2, ok2, nc = solve_with_corrections(h, net, tol, cap, refresh, check_every)
+        out.update(corr_iters=it2, corr_residual=r2, corr_ok=ok2, corrections=nc, refresh=refresh)
+    return out
 
 
 def evaluate(net: UNet, n: int = 64, scenes: int = 8, seed: int = 12345, steps: int = 30,
-             tol: float = 1e-4, log=print):
+             tol: float = 1e-4, log=print, refresh: int | None = None):
     rng = np.random.default_rng(seed)
     rows = []
     for _ in range(scenes):
         cfg = random_scene(rng, n, seed=int(rng.integers(2**31)))
         h = FlipHandle(cfg)
         h.steps(int(rng.integers(1, steps + 1)))
-        rows.append(compare_on(h, net, tol))
+        rows.append(compare_on(h, net, tol, refresh=refresh))
         h.close()
         log("  ", rows[-1])
     p = np.array([r["plain_iters"] for r in rows], float)
     q = np.array([r["nn_iters"] for r in rows], float)
-    return dict(rows=rows, plain_mean=float(p.mean()), nn_mean=float(q.mean()),
-                ratio=float(p.sum() / max(q.sum(), 1)))
+    out = dict(rows=rows, plain_mean=float(p.mean()), nn_mean=float(q.mean()),
+               ratio=float(p.sum() / max(q.sum(), 1)))
+    if refresh:
+        c = np.array([r["corr_iters"] for r in rows], float)
+        out.update(corr_mean=float(c.mean()), corr_ratio=float(p.sum() / max(c.sum(), 1)))
+    return out
 
 
-def evaluate_bench(net: UNet, stops=(1, 10, 30), tol: float = 1e-4, log=print):
+def evaluate_bench(net: UNet, stops=(1, 10, 30), tol: float = 1e-4, log=print, refresh: int | None = None,
+                   cap: int = 60000):
     """Sweeps to `tol` on the 128^3 multiphase scene after `stops` steps."""
     h = FlipHandle(configs.multiphase_128())
     rows = []
@@ -120,12 +149,15 @@ def evaluate_bench(net: UNet, stops=(1, 10, 30), tol: float = 1e-4, log=print):
     for s in stops:
         h.steps(s - done)
         done = s
-        rows.append(dict(step=s, **compare_on(h, net, tol, cap=200000, check_every=50)))
+        rows.append(dict(step=s, **compare_on(h, net, tol, cap=cap, check_every=50, refresh=refresh)))
         log("  ", rows[-1])
     h.close()
     p = sum(r["plain_iters"] for r in rows)
     q = sum(r["nn_iters"] for r in rows)
-    return dict(rows=rows, ratio=p / max(q, 1))
+    out = dict(rows=rows, ratio=p / max(q, 1))
+    if refresh:
+        out["corr_ratio"] = p / max(sum(r["corr_iters"] for r in rows), 1)
+    return out
 
 
 def main(argv=None) -> int:
@@ -148,6 +180,7 @@ def main(argv=None) -> int:
     c.add_argument("--n", type=int, default=64)
     c.add_argument("--scenes", type=int, default=4)
     c.add_argument("--bench", action="store_true")
+    c.add_argument("--refresh", type=int, default=0)
     a = ap.parse_args(argv)
 
     if a.cmd == "train":
@@ -181,7 +214,9 @@ def main(argv=None) -> int:
     if a.cmd == "compare":
         net = UNet()
         net.load_file(a.weights)
-        ev = evaluate_bench(net) if a.bench else evaluate(net, n=a.n, scenes=a.scenes)
+        r = a.refresh or None
+        ev = (evaluate_bench(net, refresh=r) if a.bench
+              else evaluate(net, n=a.n, scenes=a.scenes, refresh=r))
         print(json.dumps({k: v for k, v in ev.items() if k != "rows"}), flush=True)
         return 0
     return 1

@@ -35,6 +35,9 @@ typedef struct aq_unet_config {
   uint64_t seed;         /* weight initialisation */
   float lr, beta1, beta2, eps;  /* Adam */
   float target_scale;    /* target = p / (sigma * target_scale * (D/64)^2) */
+  int32_t infer_extent;  /* aq_flip_nn_guess restricts grids whose largest extent
+                            exceeds this by 2x per level, runs the network on the
+                            coarsest level and prolongs its pressure back (64) */
 } aq_unet_config;
 
 aq_status aq_unet_config_default(aq_unet_config* cfg);
@@ -74,8 +77,16 @@ aq_status aq_unet_train(aq_unet* net, int32_t steps, int32_t batch, int32_t trai
 aq_status aq_unet_eval(aq_unet* net, int32_t first, int32_t count, double* loss);
 
 /* Write the U-Net pressure guess for the handle's current right-hand side into its
- * pressure field (AIR cells 0). */
+ * pressure field (AIR cells 0). Grids larger than cfg.infer_extent are restricted
+ * 2x per level for the network and its output prolonged back (cell-centred
+ * trilinear); the guess p is then scaled by alpha = <p,b>/<p,Ap>, which minimises
+ * the energy-norm error ||p* - alpha p||_A (A p* = b), so in that norm it is never
+ * further from the solution than a zero guess. */
 aq_status aq_flip_nn_guess(aq_flip* flip, aq_unet* net);
+/* One U-Net correction of the current pressure: d = alpha * net(b - A p) with the
+ * same restriction, prolongation and energy-norm scaling as the guess (alpha
+ * relative to the residual equation), then p += d on FLUID cells. */
+aq_status aq_flip_nn_correct(aq_flip* flip, aq_unet* net);
 
 enum { AQ_SOLVE_FIXED = 0, AQ_SOLVE_TOL_ZERO = 1, AQ_SOLVE_TOL_NN = 2 };
 typedef struct aq_flip_solver {
@@ -83,17 +94,22 @@ typedef struct aq_flip_solver {
   double tol;            /* target FP64 relative residual */
   int32_t max_iters;     /* sweep cap per solve */
   int32_t check_every;   /* sweeps between residual checks */
-  int32_t stall_checks;  /* consecutive checks without a 1% improvement that count as a stall */
+  int32_t stall_checks;  /* consecutive checks without a 1% improvement that end a
+                            solve as stalled (FP32 Jacobi's round-off floor) */
+  int32_t nn_refresh;    /* AQ_SOLVE_TOL_NN: sweeps between U-Net corrections of the
+                            iterate (the network applied to the residual equation
+                            A d = b - A p, p += alpha d); 0 = initial guess only */
 } aq_flip_solver;
 typedef struct aq_flip_solver_stats {
   int64_t solves;         /* pressure solves since the last reset */
   int64_t sweeps;         /* Jacobi sweeps since the last reset */
-  int64_t fallbacks;      /* U-Net guesses rejected for zero-guess Jacobi */
-  int64_t capped;         /* solves that hit max_iters above tol */
+  int64_t fallbacks;      /* U-Net guesses rejected (non-finite) for zero-guess Jacobi */
+  int64_t capped;         /* solves that ended above tol (max_iters or stall) */
   int32_t last_sweeps;
   double last_residual;   /* FP64 relative residual at the end of the last solve */
   double last_guess_residual; /* residual of the last initial guess */
   int32_t last_fallback;
+  int64_t nn_corrections; /* U-Net residual corrections applied */
 } aq_flip_solver_stats;
 /* Select how aq_flip_step solves for pressure; net is required for
  * AQ_SOLVE_TOL_NN and must outlive its use. */

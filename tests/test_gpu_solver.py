@@ -45,6 +45,7 @@ def test_warm_start_reaches_tolerance_and_caps_are_reported():
     st = solver_stats(h, reset=True)
     print(st)
     assert st["solves"] == 2 and st["last_residual"] <= 1e-4
+    assert np.isfinite(st["last_guess_residual"])
     # an unreachable tolerance with a small cap is reported, and the step still completes
     attach_solver(h, "tol_zero", tol=1e-14, max_iters=40, check_every=20)
     h.step()
@@ -58,4 +59,25 @@ def test_grid_extent_must_suit_the_network():
     h = FlipHandle(configs.make_config(0, dt=0.01, pressure_iters=10, extent=(12, 16, 18), seed=3))
     with pytest.raises(Exception, match="multiples of 4"):
         attach_solver(h, "tol_nn", net=net)
+    h.close()
+
+
+def test_coarse_inference_path():
+    """A grid larger than the network's resolution: the right-hand side is
+    restricted for the network and the guess prolonged back (infer_extent 16 on a
+    32^3 grid = one level); the guess is finite, zero on AIR, and the solve from
+    it reaches the tolerance."""
+    net = UNet(seed=6, infer_extent=16)
+    h = FlipHandle(configs.make_config(32, dt=0.01, pressure_iters=20, seed=8))
+    h.step()
+    from aquarius_amd.unet import nn_guess
+    nn_guess(h, net)
+    g = h.residual()
+    p = h.get_field("pressure")
+    content = h.get_field("content")[:32, :32, :32]
+    print(f"  coarse guess residual {g:.4f}")
+    assert np.isfinite(g) and np.all(np.isfinite(p))
+    assert np.all(p[content != 1] == 0)
+    it, rel, ok = h.solve_pressure(1e-4, 50000, check_every=20)
+    assert ok and rel <= 1e-4
     h.close()

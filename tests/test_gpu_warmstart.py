@@ -1,0 +1,64 @@
+"""The trained warm start (aquarius_amd/weights/unet_warmstart_64.npz, produced by
+`python -m aquarius_amd.warmstart train`) against plain Jacobi from zero: both are
+refined to the same FP64 relative residual (1e-4), and the warm start must need
+fewer sweeps -- the gain the learned part is for. Plain Jacobi from zero is the
+judge of the error in both cases (the residual is measured, never predicted)."""
+import numpy as np
+import pytest
+
+from aquarius_amd import configs, warmstart
+from aquarius_amd.fluid import FlipHandle
+
+pytestmark = pytest.mark.gpu
+
+# Measured gains are recorded in the bench output; the bar here is a clear win.
+MIN_RATIO_64 = 1.3
+
+
+def trained_net():
+    from aquarius_amd.unet import UNet
+    if not warmstart.WEIGHTS.exists():
+        pytest.skip("no trained weights in aquarius_amd/weights")
+    net = UNet()
+    net.load_file(warmstart.WEIGHTS)
+    return net
+
+
+def test_warm_start_needs_fewer_sweeps_64():
+    net = trained_net()
+    rng = np.random.default_rng(2024)  # held-out scenes (training uses seed 0)
+    plain = warm = 0
+    for i in range(3):
+        cfg = warmstart.random_scene(rng, 64, seed=int(rng.integers(2**31)))
+        h = FlipHandle(cfg)
+        h.steps(int(rng.integers(5, 30)))
+        r = warmstart.compare_on(h, net, tol=1e-4)
+        h.close()
+        print(f"  scene {i}: plain {r['plain_iters']} sweeps, warm {r['nn_iters']} sweeps "
+              f"(guess residual {r['nn_guess_residual']:.3f})", flush=True)
+        assert r["plain_ok"] and r["nn_ok"]
+        assert r["plain_residual"] <= 1e-4 and r["nn_residual"] <= 1e-4
+        plain += r["plain_iters"]
+        warm += r["nn_iters"]
+    print(f"  total: plain {plain}, warm {warm}, ratio {plain / warm:.2f}", flush=True)
+    assert plain >= MIN_RATIO_64 * warm
+
+
+def test_warm_start_solver_mode_reaches_tolerance_128():
+    """The step-level controller on the 128^3 multiphase workload: every step's
+    pressure reaches 1e-4, and the warm start spends fewer sweeps than zero."""
+    from aquarius_amd.unet import attach_solver, solver_stats
+    net = trained_net()
+    cfg = configs.multiphase_128()
+    sweeps = {}
+    for mode in ("tol_zero", "tol_nn"):
+        h = FlipHandle(cfg)
+        attach_solver(h, mode, tol=1e-4, max_iters=100000, check_every=50, net=net if mode == "tol_nn" else None)
+        h.steps(3)
+        st = solver_stats(h)
+        h.close()
+        print(f"  {mode}: {st}", flush=True)
+        assert st["solves"] == 3 and st["capped"] == 0
+        assert st["last_residual"] <= 1e-4
+        sweeps[mode] = st["sweeps"]
+    assert sweeps["tol_nn"] < sweeps["tol_zero"]
