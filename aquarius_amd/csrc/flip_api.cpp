@@ -67,6 +67,13 @@ struct aq_flip {
   int K = 1;
   long long Np = 0;
   hipStream_t stream = nullptr;
+  // Diffusion of the phase fractions depends on neither pressure nor velocities, so
+  // inside a step its sweeps run on a second stream, concurrently with the pressure
+  // solve, projection and extrapolation (fork after the divergence pass, join before
+  // the step ends).
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool overlap = true;
   std::vector<void*> allocs;
 
   Particles P[2]{};
@@ -119,6 +126,10 @@ struct aq_flip {
     }
     for (void* p : allocs) hipFree(p);
     for (hipEvent_t e : event_pool) hipEventDestroy(e);
+    if (side) hipStreamSynchronize(side);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
+    if (side) hipStreamDestroy(side);
     if (stream) hipStreamDestroy(stream);
   }
 
@@ -254,6 +265,9 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   sp.coverZ = boundary_coverage(G.X, G.Y, ref);
 
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
   for (int b = 0; b < 2; ++b) {
     AQ_TRY(h->allocT(&h->P[b].a, (size_t)Np));
     AQ_TRY(h->allocT(&h->P[b].b, (size_t)Np));
@@ -371,18 +385,21 @@ void stage_extrapolate(aq_flip* h) {
   }
 }
 
+void diffusion_sweeps(aq_flip* h, hipStream_t s) {
+  for (int i = 0; i < h->cfg.diffusion_iters; ++i) {  // the first sweep also copies AIR cells across
+    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, s);
+    h->nf ^= 1;
+  }
+}
+
 // inStep: the grid fractions were produced by this step's P2G, so with a single
 // channel they are exactly 1 (FLUID) or 0 (AIR) and every sweep is the identity;
 // Jacobi with >= 1 sweep has already zeroed AIR pressure.
 void stage_diffusion(aq_flip* h, bool inStep) {
-  const int iters = h->cfg.diffusion_iters;
-  if (iters <= 0) return;
+  if (h->cfg.diffusion_iters <= 0) return;
   if (!inStep || h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
   if (inStep && h->K == 1) return;
-  for (int i = 0; i < iters; ++i) {  // the first sweep also copies AIR cells across
-    launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, h->stream);
-    h->nf ^= 1;
-  }
+  diffusion_sweeps(h, h->stream);
 }
 
 aq_status run_stage(aq_flip* h, int st) {
@@ -524,16 +541,34 @@ aq_status enqueue_step(aq_flip* h) {
   h->begin(AQ_STAGE_DIVERGENCE);
   launch_divergence(h->G, h->gr, h->nv, h->B, h->stream);
   h->end(AQ_STAGE_DIVERGENCE);
-  AQ_TRY(pressure_stage(h));
+  // Diffusion reads the fractions, content and active-block flags, all final after
+  // the divergence pass, and writes only the fractions, which nothing before the
+  // next step's G2P reads. (Stage timing keeps the reference's sequential order.)
+  const bool side = h->overlap && !h->timing && h->cfg.diffusion_iters > 0 && h->K > 1;
+  if (side) {
+    HIP_TRY(hipEventRecord(h->ev_fork, h->stream));
+    HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    diffusion_sweeps(h, h->side);
+    HIP_TRY(hipEventRecord(h->ev_join, h->side));
+  }
+  if (const aq_status ps = pressure_stage(h); ps != AQ_OK) {
+    if (side) hipStreamWaitEvent(h->stream, h->ev_join, 0);  // never leave the fork open
+    return ps;
+  }
   h->begin(AQ_STAGE_PROJECT);
   launch_project(h->G, h->gr, h->nv, h->hv, h->pb, h->stream);
   h->end(AQ_STAGE_PROJECT);
   h->begin(AQ_STAGE_EXTRAPOLATE);
   stage_extrapolate(h);
   h->end(AQ_STAGE_EXTRAPOLATE);
-  h->begin(AQ_STAGE_DIFFUSION);
-  stage_diffusion(h, true);
-  h->end(AQ_STAGE_DIFFUSION);
+  if (side) {
+    if (h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
+    HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+  } else {
+    h->begin(AQ_STAGE_DIFFUSION);
+    stage_diffusion(h, true);
+    h->end(AQ_STAGE_DIFFUSION);
+  }
   HIP_TRY(hipGetLastError());
   h->physicalTime += h->cfg.dt;
   return AQ_OK;
@@ -594,6 +629,7 @@ aq_status create_common(const aq_flip_config* cfg, const float* pos, const float
                         const float* frac4, long long n, aq_flip** out) {
   aq_flip* h = new aq_flip;
   if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
+  if (const char* env = std::getenv("AQFLIP_NO_OVERLAP")) h->overlap = env[0] != '1';
   aq_status st = setup(h, cfg, n);
   if (st == AQ_OK) st = check_positions(h, pos, n);
   if (st == AQ_OK) {

@@ -11,8 +11,9 @@ from aquarius_amd.fluid import FlipHandle
 
 pytestmark = pytest.mark.gpu
 
-# Measured gains are recorded in the bench output; the bar here is a clear win.
-MIN_RATIO_64 = 1.3
+# Measured gains are recorded in the bench output; the bars here are clear wins.
+MIN_RATIO_64 = 1.25            # initial guess only
+MIN_RATIO_64_CORRECTED = 2.0   # guess plus a network correction every 200 sweeps
 
 
 def trained_net():
@@ -27,33 +28,40 @@ def trained_net():
 def test_warm_start_needs_fewer_sweeps_64():
     net = trained_net()
     rng = np.random.default_rng(2024)  # held-out scenes (training uses seed 0)
-    plain = warm = 0
+    plain = warm = corr = 0
     for i in range(3):
         cfg = warmstart.random_scene(rng, 64, seed=int(rng.integers(2**31)))
         h = FlipHandle(cfg)
         h.steps(int(rng.integers(5, 30)))
-        r = warmstart.compare_on(h, net, tol=1e-4)
+        r = warmstart.compare_on(h, net, tol=1e-4, refresh=200)
         h.close()
         print(f"  scene {i}: plain {r['plain_iters']} sweeps, warm {r['nn_iters']} sweeps "
-              f"(guess residual {r['nn_guess_residual']:.3f})", flush=True)
-        assert r["plain_ok"] and r["nn_ok"]
-        assert r["plain_residual"] <= 1e-4 and r["nn_residual"] <= 1e-4
+              f"(guess residual {r['nn_guess_residual']:.3f}), corrected {r['corr_iters']} sweeps "
+              f"({r['corrections']} corrections)", flush=True)
+        assert r["plain_ok"] and r["nn_ok"] and r["corr_ok"]
+        assert max(r["plain_residual"], r["nn_residual"], r["corr_residual"]) <= 1e-4
         plain += r["plain_iters"]
         warm += r["nn_iters"]
-    print(f"  total: plain {plain}, warm {warm}, ratio {plain / warm:.2f}", flush=True)
+        corr += r["corr_iters"]
+    print(f"  total: plain {plain}, warm {warm} ({plain / warm:.2f}x), corrected {corr} "
+          f"({plain / corr:.2f}x)", flush=True)
     assert plain >= MIN_RATIO_64 * warm
+    assert plain >= MIN_RATIO_64_CORRECTED * corr
 
 
 def test_warm_start_solver_mode_reaches_tolerance_128():
-    """The step-level controller on the 128^3 multiphase workload: every step's
-    pressure reaches 1e-4, and the warm start spends fewer sweeps than zero."""
+    """The step-level controller on the 128^3 multiphase workload (warm start with
+    a network correction every 200 sweeps): every step's pressure reaches 1e-4, and
+    the warm start spends at least 2x fewer sweeps than zero."""
     from aquarius_amd.unet import attach_solver, solver_stats
     net = trained_net()
     cfg = configs.multiphase_128()
     sweeps = {}
     for mode in ("tol_zero", "tol_nn"):
         h = FlipHandle(cfg)
-        attach_solver(h, mode, tol=1e-4, max_iters=100000, check_every=50, net=net if mode == "tol_nn" else None)
+        nn = mode == "tol_nn"
+        attach_solver(h, mode, tol=1e-4, max_iters=100000, check_every=50, net=net if nn else None,
+                      nn_refresh=200 if nn else 0)
         h.steps(3)
         st = solver_stats(h)
         h.close()
@@ -61,4 +69,4 @@ def test_warm_start_solver_mode_reaches_tolerance_128():
         assert st["solves"] == 3 and st["capped"] == 0
         assert st["last_residual"] <= 1e-4
         sweeps[mode] = st["sweeps"]
-    assert sweeps["tol_nn"] < sweeps["tol_zero"]
+    assert 2 * sweeps["tol_nn"] <= sweeps["tol_zero"]
