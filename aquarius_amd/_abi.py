@@ -182,6 +182,13 @@ PROTOTYPES = {
     "aq_flip_set_solver": (C.c_int, [_P, C.POINTER(Solver), _P]),
     "aq_flip_solver_stats_get": (C.c_int, [_P, C.POINTER(SolverStats), C.c_int32]),
     "aq_unet_mfma_probe": (C.c_int, [_FP, _FP, _FP]),
+    "aq_flip_save": (C.c_int, [_P, C.c_char_p]),
+    "aq_flip_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    # spatial hashing service (include/aqflip_hash.h)
+    "aq_hash_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int64, C.POINTER(_P)]),
+    "aq_hash_destroy": (None, [_P]),
+    "aq_hash_sort_host": (C.c_int, [_P, _FP, C.c_int64, C.c_int32, _IP, _IP, _IP, _IP, _FP]),
+    "aq_hash_sort_device": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

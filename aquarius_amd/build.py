@@ -36,7 +36,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-LIB_SOURCES = ["flip_kernels.hip", "flip_api.cpp", "flip_seed.cpp", "unet_kernels.hip", "unet_api.cpp"]
+LIB_SOURCES = ["flip_kernels.hip", "flip_api.cpp", "flip_seed.cpp", "unet_kernels.hip", "unet_api.cpp",
+               "hash_api.cpp", "seed_kernels.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -105,12 +105,37 @@ struct StepParams {
   int coverX, coverY, coverZ;  // fixBoundary pair coverage
 };
 
-// ---- launchers (flip_kernels.hip) ----
+// One seeding volume as the GPU seeder walks it (seed_kernels.hip): cells in the
+// host's loop order over extents n[0..2] from origin o[0..2], loop level j running
+// along grid axis axis[j]; draws start at stream index `base`.
+struct SeedVolume {
+  int n[3], o[3], axis[3];
+  int sphere;
+  float cc[3], limit;
+  int phase;
+  uint64_t base;
+};
+struct SeedParams {
+  uint64_t seed;
+  float h, jitter;
+  int S[3];
+};
+
+// ---- launchers (flip_kernels.hip, seed_kernels.hip) ----
 void launch_fill_u32(void* p, uint32_t v, long long n, hipStream_t s);
+void launch_seed_count(const SeedVolume& v, const SeedParams& sp, long long nc, int* counts, hipStream_t s);
+void launch_seed_write(const SeedVolume& v, const SeedParams& sp, long long nc, const int* offsets,
+                       long long out0, Particles P, hipStream_t s);
+int seed_block_size();
 void launch_hash_count(const Geom& G, Particles P, long long Np, Bins B, hipStream_t s);
 void launch_scan(const Geom& G, Bins B, hipStream_t s);
 void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, long long Np, int K,
                                 Bins B, hipStream_t s);
+// Standalone spatial hashing (records of `stride` floats, position first).
+void launch_hash_records(const Geom& G, const float* pos, long long n, int stride, Bins B, hipStream_t s);
+void launch_scatter_segsort(const Geom& G, long long n, Bins B, hipStream_t s);
+void launch_hash_outputs(const Geom& G, long long n, int stride, Bins B, const float* rec, int* start, int* end,
+                         int* sorted_hash, float* sorted_rec, hipStream_t s);
 void launch_p2g(const Geom& G, Particles P, long long Np, int K, Bins B, Grid gr, int nv,
                 int hv, int nf, StepParams sp, hipStream_t s);
 void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int doGravity,

@@ -625,16 +625,138 @@ aq_status graph_step(aq_flip* h) {
   return AQ_OK;
 }
 
+// ---- particle seeding on the GPU ------------------------------------------
+// The volumes of a scene as the GPU seeder walks them, with per-block output
+// offsets from a counting pass (seed_kernels.hip); identical to seed_scene's output.
+struct SeedPlan {
+  SeedParams sp{};
+  std::vector<SeedVolume> vols;
+  std::vector<long long> nc;                // candidates per volume (cells x 8)
+  std::vector<std::vector<int>> offsets;    // per volume: exclusive prefix of block counts
+  std::vector<long long> first;             // per volume: index of its first particle
+  long long total = 0;
+};
+
+bool in_unit_cube(const float c[3]) {
+  for (int a = 0; a < 3; ++a)
+    if (c[a] < 0 || c[a] > 1) return false;
+  return true;
+}
+
+aq_status plan_seeding(const aq_flip_config& cfg, SeedPlan* plan) {
+  const int S[3] = {cfg.size[0], cfg.size[1], cfg.size[2]};
+  const float h = cfg.grid_physical_size / (float)cfg.size[1];
+  plan->sp.seed = cfg.seed;
+  plan->sp.h = h;
+  plan->sp.jitter = cfg.particle_jitter;
+  for (int a = 0; a < 3; ++a) plan->sp.S[a] = S[a];
+  uint64_t draws = 0;
+  for (int i = 0; i < cfg.n_volumes; ++i) {
+    const aq_flip_volume& v = cfg.volumes[i];
+    SeedVolume sv{};
+    sv.phase = v.phase;
+    if (v.shape == AQ_SHAPE_BOX) {
+      if (!in_unit_cube(v.box_min) || !in_unit_cube(v.box_max)) continue;
+      // createSquareFluid's loops: z, then x, then y, each from (int)(min*S) while (float)i < max*S
+      const int order[3] = {2, 0, 1};
+      for (int j = 0; j < 3; ++j) {
+        const int a = order[j];
+        const int lo = (int)(v.box_min[a] * (float)S[a]);
+        int cnt = 0;
+        for (int c = lo; (float)c < v.box_max[a] * (float)S[a]; ++c) ++cnt;
+        sv.n[j] = cnt;
+        sv.o[j] = lo;
+        sv.axis[j] = a;
+      }
+    } else if (v.shape == AQ_SHAPE_SPHERE) {
+      if (!in_unit_cube(v.center)) continue;
+      // createSphereFluid visits every cell: y, then x, then z
+      const int order[3] = {1, 0, 2};
+      for (int j = 0; j < 3; ++j) {
+        sv.n[j] = S[order[j]];
+        sv.o[j] = 0;
+        sv.axis[j] = order[j];
+      }
+      sv.sphere = 1;
+      for (int a = 0; a < 3; ++a) sv.cc[a] = (v.center[a] * (float)S[a]) * h;
+      sv.limit = (v.radius * h) * (float)S[1];
+    } else {
+      continue;
+    }
+    sv.base = draws;
+    const long long cells = (long long)std::max(sv.n[0], 0) * std::max(sv.n[1], 0) * std::max(sv.n[2], 0);
+    draws += (uint64_t)cells * 24u;
+    if (cells == 0) continue;
+    plan->vols.push_back(sv);
+    plan->nc.push_back(cells * 8);
+  }
+  if (plan->vols.empty()) return AQ_OK;  // nothing to seed (no device work)
+  // counting pass
+  hipStream_t s = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const int B = seed_block_size();
+  long long maxb = 1;
+  for (long long nc : plan->nc) maxb = std::max(maxb, (nc + B - 1) / B);
+  int* d_counts = nullptr;
+  hipError_t e = hipMalloc((void**)&d_counts, (size_t)maxb * sizeof(int));
+  for (size_t i = 0; i < plan->vols.size() && e == hipSuccess; ++i) {
+    const long long nb = (plan->nc[i] + B - 1) / B;
+    launch_seed_count(plan->vols[i], plan->sp, plan->nc[i], d_counts, s);
+    std::vector<int> counts((size_t)nb);
+    e = hipMemcpyAsync(counts.data(), d_counts, (size_t)nb * sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    std::vector<int> off((size_t)nb);
+    long long run = 0;
+    for (long long b = 0; b < nb; ++b) {
+      off[(size_t)b] = (int)run;
+      run += counts[(size_t)b];
+    }
+    plan->first.push_back(plan->total);
+    plan->total += run;
+    plan->offsets.push_back(std::move(off));
+  }
+  if (d_counts) hipFree(d_counts);
+  hipStreamDestroy(s);
+  if (e != hipSuccess) return fail(AQ_EHIP, std::string("seeding: ") + hipGetErrorString(e));
+  if (plan->total > (1ll << 31) - 1) return fail(AQ_EINVAL, "too many particles");
+  return AQ_OK;
+}
+
+// Write the planned particles into the handle's current particle buffer.
+aq_status write_seeded(aq_flip* h, const SeedPlan& plan) {
+  long long maxb = 1;
+  for (const auto& o : plan.offsets) maxb = std::max<long long>(maxb, (long long)o.size());
+  int* d_off = nullptr;
+  HIP_TRY(hipMalloc((void**)&d_off, (size_t)maxb * sizeof(int)));
+  hipError_t e = hipSuccess;
+  for (size_t i = 0; i < plan.vols.size() && e == hipSuccess; ++i) {
+    e = hipMemcpyAsync(d_off, plan.offsets[i].data(), plan.offsets[i].size() * sizeof(int),
+                       hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+      launch_seed_write(plan.vols[i], plan.sp, plan.nc[i], d_off, plan.first[i], h->P[h->cur], h->stream);
+      e = hipStreamSynchronize(h->stream);  // d_off is reused by the next volume
+    }
+  }
+  hipFree(d_off);
+  if (e != hipSuccess) return fail(AQ_EHIP, std::string("seeding: ") + hipGetErrorString(e));
+  return AQ_OK;
+}
+
 aq_status create_common(const aq_flip_config* cfg, const float* pos, const float* vel,
-                        const float* frac4, long long n, aq_flip** out) {
+                        const float* frac4, long long n, aq_flip** out, const SeedPlan* plan = nullptr) {
   aq_flip* h = new aq_flip;
   if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
   if (const char* env = std::getenv("AQFLIP_NO_OVERLAP")) h->overlap = env[0] != '1';
   aq_status st = setup(h, cfg, n);
-  if (st == AQ_OK) st = check_positions(h, pos, n);
-  if (st == AQ_OK) {
-    h->K = channels_for(*cfg, frac4, n);
-    st = upload_particles(h, pos, vel, frac4);
+  if (plan) {  // seeded on the device: positions are clamped into the domain by construction
+    h->K = channels_for(*cfg, nullptr, 0);
+    if (st == AQ_OK) st = write_seeded(h, *plan);
+  } else {
+    if (st == AQ_OK) st = check_positions(h, pos, n);
+    if (st == AQ_OK) {
+      h->K = channels_for(*cfg, frac4, n);
+      st = upload_particles(h, pos, vel, frac4);
+    }
   }
   if (st == AQ_OK) {
     // Fluid::init: fixBoundary, then transferToGrid (Fluid_3D_FLIP.cu:485,500).
@@ -777,10 +899,10 @@ aq_status aq_flip_seed_particles(const aq_flip_config* cfg, float* pos, float* f
 aq_status aq_flip_create(const aq_flip_config* cfg, aq_flip** out) {
   AQ_TRY(validate(cfg));
   if (!out) return fail(AQ_EINVAL, "null out");
-  std::vector<float> p, f;
-  const int64_t n = seed_scene(*cfg, &p, &f);
-  if (n <= 0) return fail(AQ_EINVAL, "the configured volumes produce no particles");
-  return create_common(cfg, p.data(), nullptr, f.data(), n, out);
+  SeedPlan plan;
+  AQ_TRY(plan_seeding(*cfg, &plan));
+  if (plan.total <= 0) return fail(AQ_EINVAL, "the configured volumes produce no particles");
+  return create_common(cfg, nullptr, nullptr, nullptr, plan.total, out, &plan);
 }
 
 aq_status aq_flip_create_with_particles(const aq_flip_config* cfg, const float* pos,
@@ -1119,6 +1241,168 @@ aq_status aq_flip_solve_pressure(aq_flip* h, double tol, int32_t max_iters, int3
   if (iters_out) *iters_out = it;
   if (rel_out) *rel_out = rel;
   if (!(rel <= tol)) return fail(AQ_ESTALL, "pressure solve reached its iteration cap");
+  return AQ_OK;
+}
+
+// ---- snapshots ---------------------------------------------------------------
+extern "C++" {
+namespace {
+
+constexpr char kSnapMagic[8] = {'A', 'Q', 'F', 'L', 'I', 'P', 'S', 'N'};
+
+// Every device array that carries state between calls, tagged in a fixed order.
+// Ping-pong pairs contribute their current buffer only.
+struct SnapArray {
+  uint32_t tag;
+  void* ptr;
+  size_t bytes;
+};
+std::vector<SnapArray> snapshot_arrays(aq_flip* h) {
+  const Geom& G = h->G;
+  const size_t nc = (size_t)G.ncell, np = (size_t)h->Np, nh = (size_t)G.ncells_hash;
+  Grid& gr = h->gr;
+  std::vector<SnapArray> a;
+  const Particles& P = h->P[h->cur];
+  a.push_back({1, P.a, np * sizeof(float4)});
+  a.push_back({2, P.b, np * sizeof(float4)});
+  if (h->K > 2) a.push_back({3, P.c, np * sizeof(float4)});
+  a.push_back({10, gr.content, nc});
+  for (int c = 0; c < 3; ++c) a.push_back({11u + c, gr.vel[c], nc * 4});
+  for (int c = 0; c < 3; ++c) a.push_back({14u + c, gr.nvel[h->nv][c], nc * 4});
+  a.push_back({17, gr.hasvel[h->hv], nc});
+  a.push_back({18, gr.p[h->pb], nc * 4});
+  a.push_back({19, gr.div, nc * 4});
+  a.push_back({20, gr.rhs, nc * 4});
+  a.push_back({21, gr.density, nc * 4});
+  for (int c = 0; c < h->K; ++c) a.push_back({30u + c, gr.vf[c], nc * 4});
+  for (int c = 0; c < h->K; ++c) a.push_back({40u + c, gr.nvf[h->nf][c], nc * 4});
+  a.push_back({50, gr.active, ((nh + 255) / 256 + 1) * sizeof(uint32_t)});
+  a.push_back({60, h->B.offsets, (nh + 1) * sizeof(int)});
+  a.push_back({61, h->B.hash, np * sizeof(int)});
+  a.push_back({62, h->B.perm, np * sizeof(int)});
+  return a;
+}
+
+// Configuration fields in a fixed order (independent of the struct's padding).
+template <typename IO>
+void config_fields(IO&& io, aq_flip_config& c) {
+  for (int a = 0; a < 3; ++a) io(c.size[a]);
+  io(c.dt), io(c.flip_coeff), io(c.particle_jitter), io(c.grid_physical_size);
+  io(c.pressure_iters), io(c.diffusion_iters), io(c.phase_count);
+  for (int a = 0; a < 3; ++a) io(c.gravity[a]);
+  for (int a = 0; a < 4; ++a) io(c.phase_densities[a]);
+  io(c.diffusion_coeff), io(c.pressure_init), io(c.ref_boundary_coverage), io(c.seed), io(c.n_volumes);
+  for (int i = 0; i < AQ_MAX_VOLUMES; ++i) {
+    aq_flip_volume& v = c.volumes[i];
+    io(v.shape);
+    for (int a = 0; a < 3; ++a) io(v.box_min[a]);
+    for (int a = 0; a < 3; ++a) io(v.box_max[a]);
+    for (int a = 0; a < 3; ++a) io(v.center[a]);
+    io(v.radius), io(v.phase);
+  }
+}
+
+}  // namespace
+}  // extern "C++"
+
+aq_status aq_flip_save(aq_flip* h, const char* path) {
+  if (!h || !path) return fail(AQ_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(AQ_EINVAL, std::string("cannot open ") + path + " for writing");
+  bool ok = true;
+  auto put = [&](const void* p, size_t n) { ok = ok && std::fwrite(p, 1, n, f) == n; };
+  auto io = [&](auto& v) { put(&v, sizeof(v)); };
+  put(kSnapMagic, 8);
+  uint32_t version = AQ_SNAPSHOT_VERSION;
+  io(version);
+  aq_flip_config cfg = h->cfg;
+  config_fields(io, cfg);
+  int64_t np = h->Np;
+  int32_t K = h->K, nsec = 0;
+  float t = h->physicalTime;
+  io(np), io(K), io(t);
+  const std::vector<SnapArray> arrays = snapshot_arrays(h);
+  nsec = (int32_t)arrays.size();
+  io(nsec);
+  std::vector<char> buf;
+  for (const SnapArray& a : arrays) {
+    uint32_t tag = a.tag;
+    uint64_t bytes = a.bytes;
+    io(tag), io(bytes);
+    buf.resize(a.bytes);
+    if (hipMemcpy(buf.data(), a.ptr, a.bytes, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+    put(buf.data(), a.bytes);
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(AQ_EHIP, std::string("writing the snapshot ") + path + " failed");
+  return AQ_OK;
+}
+
+aq_status aq_flip_load(const char* path, aq_flip** out) {
+  if (!path || !out) return fail(AQ_EINVAL, "null argument");
+  *out = nullptr;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(AQ_EINVAL, std::string("cannot open ") + path);
+  bool ok = true;
+  auto get = [&](void* p, size_t n) { ok = ok && std::fread(p, 1, n, f) == n; };
+  auto io = [&](auto& v) { get(&v, sizeof(v)); };
+  auto bail = [&](aq_status st, const std::string& msg) {
+    std::fclose(f);
+    return fail(st, msg);
+  };
+  char magic[8];
+  get(magic, 8);
+  uint32_t version = 0;
+  io(version);
+  if (!ok || std::memcmp(magic, kSnapMagic, 8) != 0) return bail(AQ_EINVAL, "not a libaqflip snapshot");
+  if (version != AQ_SNAPSHOT_VERSION)
+    return bail(AQ_EINVAL, "snapshot format version " + std::to_string(version) + " is not supported");
+  aq_flip_config cfg{};
+  config_fields(io, cfg);
+  int64_t np = 0;
+  int32_t K = 0, nsec = 0;
+  float t = 0.f;
+  io(np), io(K), io(t), io(nsec);
+  if (!ok) return bail(AQ_EINVAL, "truncated snapshot header");
+  if (validate(&cfg) != AQ_OK) return bail(AQ_EINVAL, std::string("snapshot configuration: ") + aq_last_error());
+  if (np <= 0 || np > (1ll << 31) - 1 || K < 1 || K > 4) return bail(AQ_EINVAL, "corrupt snapshot state");
+  aq_flip* h = new aq_flip;
+  if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
+  if (const char* env = std::getenv("AQFLIP_NO_OVERLAP")) h->overlap = env[0] != '1';
+  aq_status st = setup(h, &cfg, np);
+  if (st != AQ_OK) {
+    delete h;
+    return bail(st, aq_last_error());
+  }
+  h->K = K;
+  h->physicalTime = t;
+  const std::vector<SnapArray> arrays = snapshot_arrays(h);
+  if ((size_t)nsec != arrays.size()) {
+    delete h;
+    return bail(AQ_EINVAL, "snapshot sections do not match its configuration");
+  }
+  std::vector<char> buf;
+  for (const SnapArray& a : arrays) {
+    uint32_t tag = 0;
+    uint64_t bytes = 0;
+    io(tag), io(bytes);
+    if (!ok || tag != a.tag || bytes != a.bytes) {
+      delete h;
+      return bail(AQ_EINVAL, "corrupt snapshot section " + std::to_string(a.tag));
+    }
+    buf.resize(a.bytes);
+    get(buf.data(), a.bytes);
+    if (!ok || hipMemcpy(a.ptr, buf.data(), a.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      delete h;
+      return bail(AQ_EINVAL, "truncated snapshot section " + std::to_string(a.tag));
+    }
+  }
+  std::fclose(f);
+  // Pressure buffers: only the current one was saved, so neither is known to hold 0
+  // on every AIR cell; the next sweeps write them fully (same values either way).
+  h->air_clean[0] = h->air_clean[1] = false;
+  *out = h;
   return AQ_OK;
 }
 

@@ -279,6 +279,68 @@ void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, lon
     hipLaunchKernelGGL(k_gather<2>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm);
 }
 
+// ------------------------------------------------------------------ standalone spatial hashing
+// performSpatialHashing2 (Fluid/Fluid_3D_common.cuh:28-56) for any particle layout:
+// positions are the first three floats of records of `stride` floats.
+__global__ void k_hash_records(Geom G, const float* __restrict__ pos, long long n, int stride,
+                               int* __restrict__ hash, int* __restrict__ rank,
+                               int* __restrict__ count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  int hh = 0;
+  if (valid) {
+    const float* p = pos + i * stride;
+    hh = cell_hash(G, p[0], p[1], p[2]);
+  }
+  int r;
+  bin_particle(hh, valid, count, r);
+  if (valid) {
+    hash[i] = hh;
+    rank[i] = r;
+  }
+}
+void launch_hash_records(const Geom& G, const float* pos, long long n, int stride, Bins B, hipStream_t s) {
+  hipLaunchKernelGGL(k_hash_records, dim3(nblocks(n)), dim3(kBlock), 0, s, G, pos, n, stride, B.hash,
+                     B.rank, B.count);
+}
+void launch_scatter_segsort(const Geom& G, long long n, Bins B, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks(n)), dim3(kBlock), 0, s, B.hash, B.rank, n, B.offsets,
+                     B.perm);
+  hipLaunchKernelGGL(k_segsort, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, B.offsets,
+                     G.ncells_hash, B.perm);
+}
+// findCellStartEndImpl (Fluid/Fluid_kernels.cuh:3-22) from the offsets: first and
+// last (inclusive) sorted slot of every cell, -1 for empty cells.
+__global__ void k_cell_ranges(int ncells, const int* __restrict__ offsets, int* __restrict__ start,
+                              int* __restrict__ end) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int s = offsets[c], e = offsets[c + 1];
+  if (start) start[c] = e > s ? s : -1;
+  if (end) end[c] = e > s ? e - 1 : -1;
+}
+// Sorted hashes and, optionally, the records in sorted order (applySortImpl,
+// Fluid/Fluid_3D_kernels.cuh:95-102).
+__global__ void k_sorted_outputs(long long n, const int* __restrict__ perm, const int* __restrict__ hash,
+                                 int* __restrict__ sorted_hash, const float* __restrict__ rec,
+                                 float* __restrict__ sorted_rec, int stride) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int j = perm[k];
+  if (sorted_hash) sorted_hash[k] = hash[j];
+  if (sorted_rec)
+    for (int f = 0; f < stride; ++f) sorted_rec[k * stride + f] = rec[(long long)j * stride + f];
+}
+void launch_hash_outputs(const Geom& G, long long n, int stride, Bins B, const float* rec, int* start, int* end,
+                         int* sorted_hash, float* sorted_rec, hipStream_t s) {
+  if (start || end)
+    hipLaunchKernelGGL(k_cell_ranges, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G.ncells_hash,
+                       B.offsets, start, end);
+  if ((sorted_hash || sorted_rec) && n > 0)
+    hipLaunchKernelGGL(k_sorted_outputs, dim3(nblocks(n)), dim3(kBlock), 0, s, n, B.perm, B.hash,
+                       sorted_hash, rec, sorted_rec, stride);
+}
+
 // ------------------------------------------------------------------ P2G
 // Fuses resetCellImpl (Fluid_3D_FLIP.cu:287-302), transferToCellAccumPhase (:8-69),
 // transferToCellDividePhase (:72-116) and transferVolumeFractionsToCell (:118-165).

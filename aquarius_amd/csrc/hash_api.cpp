@@ -1,0 +1,183 @@
+// hash_api.cpp -- the spatial hashing service (include/aqflip_hash.h): the FLIP
+// step's hash, stable counting sort and cell ranges on caller-provided records.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/aqflip_hash.h"
+#include "aqflip_internal.h"
+#include "flip_view.h"
+
+using namespace aqf;
+
+namespace {
+
+aq_status hfail(aq_status s, const std::string& m) {
+  set_last_error(m);
+  return s;
+}
+
+#define HHIP(expr)                                                                            \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return hfail(e_ == hipErrorOutOfMemory ? AQ_ENOMEM : AQ_EHIP,                           \
+                   std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+  } while (0)
+
+#define HTRY(expr)               \
+  do {                           \
+    aq_status s_ = (expr);       \
+    if (s_ != AQ_OK) return s_;  \
+  } while (0)
+
+}  // namespace
+
+struct aq_hash {
+  Geom G{};
+  hipStream_t stream = nullptr;
+  Bins B{};
+  long long cap = 0;  // particles the per-particle buffers hold
+  // staging for the host entry point
+  float *d_rec = nullptr, *d_sorted = nullptr;
+  int *d_start = nullptr, *d_end = nullptr, *d_shash = nullptr;
+  size_t rec_cap = 0;
+  std::vector<void*> fixed, per_particle, staging;
+
+  ~aq_hash() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void* p : staging) hipFree(p);
+    for (void* p : per_particle) hipFree(p);
+    for (void* p : fixed) hipFree(p);
+    if (stream) hipStreamDestroy(stream);
+  }
+  template <typename T>
+  aq_status alloc(T** p, size_t n, std::vector<void*>& list) {
+    HHIP(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
+    list.push_back(*p);
+    return AQ_OK;
+  }
+  aq_status ensure(long long n) {
+    if (n <= cap) return AQ_OK;
+    HHIP(hipStreamSynchronize(stream));
+    for (void* p : per_particle) hipFree(p);
+    per_particle.clear();
+    HTRY(alloc(&B.hash, (size_t)n, per_particle));
+    HTRY(alloc(&B.rank, (size_t)n, per_particle));
+    HTRY(alloc(&B.perm, (size_t)n, per_particle));
+    cap = n;
+    return AQ_OK;
+  }
+  aq_status ensure_staging(long long n, int stride) {
+    const size_t need = (size_t)std::max<long long>(n, 1) * stride;
+    if (need <= rec_cap) return AQ_OK;
+    HHIP(hipStreamSynchronize(stream));
+    for (void* p : staging) hipFree(p);
+    staging.clear();
+    HTRY(alloc(&d_rec, need, staging));
+    HTRY(alloc(&d_sorted, need, staging));
+    HTRY(alloc(&d_shash, (size_t)std::max<long long>(n, 1), staging));
+    rec_cap = need;
+    return AQ_OK;
+  }
+
+  // The whole pipeline on device records; outputs are device pointers (may be null).
+  aq_status run(const float* rec, long long n, int stride, int* perm, int* shash, int* start, int* end,
+                float* sorted) {
+    HTRY(ensure(n));
+    const int ncells = G.ncells_hash;
+    launch_fill_u32(B.count, 0u, ncells, stream);
+    if (n > 0) launch_hash_records(G, rec, n, stride, B, stream);
+    launch_scan(G, B, stream);
+    if (n > 0) launch_scatter_segsort(G, n, B, stream);
+    launch_hash_outputs(G, n, stride, B, rec, start, end, shash, sorted, stream);
+    if (perm && n > 0) HHIP(hipMemcpyAsync(perm, B.perm, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
+    HHIP(hipGetLastError());
+    return AQ_OK;
+  }
+};
+
+namespace {
+
+aq_status check_args(const aq_hash* h, const void* rec, long long n, int stride) {
+  if (!h) return hfail(AQ_EINVAL, "null handle");
+  if (n < 0 || n > 0x7fffffffLL) return hfail(AQ_EINVAL, "particle count must be in [0, 2^31)");
+  if (n > 0 && !rec) return hfail(AQ_EINVAL, "null records");
+  if (stride < 3) return hfail(AQ_EINVAL, "stride must be >= 3 floats");
+  return AQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+aq_status aq_hash_create(int32_t sx, int32_t sy, int32_t sz, float cell_size, int64_t capacity,
+                         aq_hash** out) {
+  if (!out) return hfail(AQ_EINVAL, "null output");
+  *out = nullptr;
+  if (sx < 1 || sy < 1 || sz < 1) return hfail(AQ_EINVAL, "grid sizes must be >= 1");
+  if ((long long)sx * sy * sz >= 0x7fffffffLL) return hfail(AQ_EINVAL, "grid too large");
+  if (!(cell_size > 0) || !std::isfinite(cell_size)) return hfail(AQ_EINVAL, "cell_size must be positive");
+  if (capacity < 0) return hfail(AQ_EINVAL, "negative capacity");
+  aq_hash* h = new aq_hash;
+  Geom& G = h->G;
+  G.X = sx, G.Y = sy, G.Z = sz;
+  G.ncells_hash = sx * sy * sz;
+  G.h = cell_size;
+  G.inv_h = 1.0 / (double)cell_size;
+  G.divZ = make_fastdiv((uint32_t)sz);
+  G.divYZ = make_fastdiv((uint32_t)(sy * sz));
+  auto bail = [&](aq_status s) {
+    delete h;
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(hfail(AQ_EHIP, "stream creation failed"));
+  const int nch = G.ncells_hash;
+  aq_status st = h->alloc(&h->B.count, (size_t)nch, h->fixed);
+  if (st == AQ_OK) st = h->alloc(&h->B.offsets, (size_t)nch + 1, h->fixed);
+  if (st == AQ_OK) st = h->alloc(&h->B.partials, (size_t)(nch + 4095) / 4096 + 1, h->fixed);
+  if (st == AQ_OK) st = h->alloc(&h->d_start, (size_t)nch, h->fixed);
+  if (st == AQ_OK) st = h->alloc(&h->d_end, (size_t)nch, h->fixed);
+  if (st == AQ_OK) st = h->ensure(std::max<int64_t>(capacity, 1));
+  if (st != AQ_OK) return bail(st);
+  *out = h;
+  return AQ_OK;
+}
+
+void aq_hash_destroy(aq_hash* h) { delete h; }
+
+aq_status aq_hash_sort_host(aq_hash* h, const float* rec, int64_t n, int32_t stride, int32_t* perm,
+                            int32_t* shash, int32_t* start, int32_t* end, float* sorted) {
+  HTRY(check_args(h, rec, n, stride));
+  for (long long i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(rec[i * stride + a])) return hfail(AQ_EINVAL, "non-finite particle position");
+  HTRY(h->ensure_staging(n, stride));
+  const size_t bytes = (size_t)n * stride * sizeof(float);
+  if (n > 0) HHIP(hipMemcpyAsync(h->d_rec, rec, bytes, hipMemcpyHostToDevice, h->stream));
+  HTRY(h->run(h->d_rec, n, stride, nullptr, shash ? h->d_shash : nullptr, start ? h->d_start : nullptr,
+              end ? h->d_end : nullptr, sorted ? h->d_sorted : nullptr));
+  const size_t cb = (size_t)h->G.ncells_hash * sizeof(int);
+  if (perm && n > 0) HHIP(hipMemcpyAsync(perm, h->B.perm, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  if (shash && n > 0) HHIP(hipMemcpyAsync(shash, h->d_shash, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  if (start) HHIP(hipMemcpyAsync(start, h->d_start, cb, hipMemcpyDeviceToHost, h->stream));
+  if (end) HHIP(hipMemcpyAsync(end, h->d_end, cb, hipMemcpyDeviceToHost, h->stream));
+  if (sorted && n > 0) HHIP(hipMemcpyAsync(sorted, h->d_sorted, bytes, hipMemcpyDeviceToHost, h->stream));
+  HHIP(hipStreamSynchronize(h->stream));
+  return AQ_OK;
+}
+
+aq_status aq_hash_sort_device(aq_hash* h, const float* d_rec, int64_t n, int32_t stride, int32_t* d_perm,
+                              int32_t* d_shash, int32_t* d_start, int32_t* d_end, float* d_sorted) {
+  HTRY(check_args(h, d_rec, n, stride));
+  if (d_sorted && d_sorted == d_rec) return hfail(AQ_EINVAL, "sorted records must not alias the input");
+  HTRY(h->run(d_rec, n, stride, d_perm, d_shash, d_start, d_end, d_sorted));
+  HHIP(hipStreamSynchronize(h->stream));
+  return AQ_OK;
+}
+
+}  // extern "C"

@@ -56,6 +56,23 @@ class FlipHandle:
         self.h = h
         self.size = tuple(cfg.size)
 
+    def save(self, path) -> None:
+        """Checkpoint (aq_flip_save): everything a later step or export reads."""
+        check(self.lib.aq_flip_save(self.h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path, cfg: Config | None = None) -> "FlipHandle":
+        """Resume from a checkpoint (aq_flip_load)."""
+        self = cls.__new__(cls)
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.aq_flip_load(str(path).encode(), C.byref(h)))
+        self.h = h
+        self.cfg = cfg
+        with open(path, "rb") as f:  # grid extents: the first configuration fields
+            self.size = tuple(int(v) for v in np.frombuffer(f.read(24)[12:24], "<i4"))
+        return self
+
     def close(self) -> None:
         if self.h:
             self.lib.aq_flip_destroy(self.h)

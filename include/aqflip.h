@@ -208,6 +208,18 @@ double aq_flip_physical_time(const aq_flip* h);
 aq_status aq_flip_get_center(const aq_flip* h, float* center);
 float aq_flip_container_size(const aq_flip* h);
 
+/* Checkpoint and resume (no counterpart in the reference, which has none). The
+ * file is a versioned little-endian snapshot: magic "AQFLIPSN", format version,
+ * the configuration field by field, the step state (particle count, phase channels,
+ * physical time, ping-pong selectors) and tagged sections holding the particles in
+ * sorted order, every grid field in the library's padded layout and the cell bins
+ * -- all that a later aq_flip_step or export reads. aq_flip_load returns a handle
+ * whose steps and exports are bit-identical to those of the saved one. The solver
+ * mode (aq_flip_set_solver) is not part of the snapshot. */
+#define AQ_SNAPSHOT_VERSION 1
+aq_status aq_flip_save(aq_flip* h, const char* path);
+aq_status aq_flip_load(const char* path, aq_flip** out);
+
 /* Thread-local message of the last failing call on this thread. */
 const char* aq_last_error(void);
 

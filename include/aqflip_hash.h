@@ -1,0 +1,61 @@
+/* aqflip_hash.h -- spatial hashing service of libaqflip (C ABI).
+ *
+ * The reference sorts particles by grid cell with performSpatialHashing2
+ * (Fluid/Fluid_3D_common.cuh:28-56), shared by FLIP (Fluid/Fluid_3D_FLIP.cu:364),
+ * PBF (Fluid/Fluid_3D_PBF.cu:474,483,491), PCISPH (Fluid/Fluid_3D_PCISPH.cu:420,434)
+ * and the mesher's SDF grid (Rendering/Renderer3D/Mesher.cuh:155). This is the same
+ * operation as a standalone handle: the cell hash of calcHashImpl
+ * (Fluid/Fluid_3D_kernels.cuh:8-27), a stable sort by hash (sort_by_key with the
+ * identity indices, Fluid/Fluid_3D_common.cuh:31-35; stable by input index), the
+ * reordered records (applySortImpl, Fluid/Fluid_3D_kernels.cuh:95-102) and the cell
+ * ranges (findCellStartEndImpl, Fluid/Fluid_kernels.cuh:3-22).
+ *
+ * hash = x*(size_y*size_z) + y*size_z + z with x = (int)(px / cell_size) (truncating,
+ * IEEE division), clamped to the grid on every axis (the reference indexes out of
+ * bounds there). Records are `stride` floats whose first three are the position:
+ * stride 3 for packed positions, 12 for the reference's 48-byte Particle.
+ *
+ * Threading: one handle per thread; calls are synchronous at return.
+ */
+#ifndef AQFLIP_HASH_H
+#define AQFLIP_HASH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "aqflip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aq_hash aq_hash;
+
+/* Grid of size_x*size_y*size_z cells of edge cell_size. capacity: expected particle
+ * count (the handle grows on demand). */
+aq_status aq_hash_create(int32_t size_x, int32_t size_y, int32_t size_z, float cell_size,
+                         int64_t capacity, aq_hash** out);
+void aq_hash_destroy(aq_hash* h);
+
+/* Host arrays. records: n records of `stride` floats. Outputs (each may be NULL):
+ *   perm[n]             sorted slot -> input index (the sorted particleIndices)
+ *   sorted_hash[n]      the sorted particleHashes
+ *   cell_start[cells], cell_end[cells]  first / last (inclusive) sorted slot of each
+ *                       cell, -1 when the cell is empty (cells = size_x*size_y*size_z)
+ *   sorted_records[n*stride]  the records in sorted order
+ * Positions must be finite. */
+aq_status aq_hash_sort_host(aq_hash* h, const float* records, int64_t n, int32_t stride,
+                            int32_t* perm, int32_t* sorted_hash, int32_t* cell_start,
+                            int32_t* cell_end, float* sorted_records);
+
+/* The same on device memory allocated by the caller (hipMalloc); complete at return.
+ * A non-finite position hashes to a clamped cell instead of being rejected. */
+aq_status aq_hash_sort_device(aq_hash* h, const float* d_records, int64_t n, int32_t stride,
+                              int32_t* d_perm, int32_t* d_sorted_hash, int32_t* d_cell_start,
+                              int32_t* d_cell_end, float* d_sorted_records);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AQFLIP_HASH_H */

@@ -15,7 +15,8 @@ from aquarius_amd import _abi, configs
 from oracle import binding as ob
 
 ROOT = Path(__file__).resolve().parent.parent
-HEADERS = [ROOT / "include" / "aqflip.h", ROOT / "include" / "aqflip_unet.h"]
+HEADERS = [ROOT / "include" / "aqflip.h", ROOT / "include" / "aqflip_unet.h",
+           ROOT / "include" / "aqflip_hash.h"]
 
 
 def declared_functions():
