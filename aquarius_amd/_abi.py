@@ -128,7 +128,7 @@ class SolverStats(C.Structure):
     ]
 
 
-AQ_SOLVE_FIXED, AQ_SOLVE_TOL_ZERO, AQ_SOLVE_TOL_NN = 0, 1, 2
+AQ_SOLVE_FIXED, AQ_SOLVE_TOL_ZERO, AQ_SOLVE_TOL_NN, AQ_SOLVE_MGPCG = 0, 1, 2, 3
 
 # name -> (restype, argtypes); every symbol declared in include/aqflip.h and
 # include/aqflip_unet.h
@@ -182,6 +182,8 @@ PROTOTYPES = {
     "aq_flip_set_solver": (C.c_int, [_P, C.POINTER(Solver), _P]),
     "aq_flip_solver_stats_get": (C.c_int, [_P, C.POINTER(SolverStats), C.c_int32]),
     "aq_unet_mfma_probe": (C.c_int, [_FP, _FP, _FP]),
+    "aq_flip_solve_pressure_mgpcg": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                               C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "aq_flip_save": (C.c_int, [_P, C.c_char_p]),
     "aq_flip_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     # spatial hashing service (include/aqflip_hash.h)

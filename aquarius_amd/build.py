@@ -37,7 +37,7 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["flip_kernels.hip", "flip_api.cpp", "flip_seed.cpp", "unet_kernels.hip", "unet_api.cpp",
-               "hash_api.cpp", "seed_kernels.hip"]
+               "hash_api.cpp", "seed_kernels.hip", "mgpcg.hip"]
 
 
 def _run(cmd: list[str]) -> None:

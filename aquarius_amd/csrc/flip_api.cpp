@@ -19,6 +19,7 @@
 #include "../../include/aqflip_unet.h"
 #include "aqflip_internal.h"
 #include "flip_view.h"
+#include "mgpcg.h"
 
 namespace aqf {
 int64_t seed_scene(const aq_flip_config& cfg, std::vector<float>* pos, std::vector<float>* frac);
@@ -113,6 +114,14 @@ struct aq_flip {
   };
   std::vector<StepGraph> graphs;
   bool use_graphs = true;
+  // Multigrid-preconditioned CG (allocated on first use).
+  struct MgState {
+    MgHierarchy H;
+    double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *d = nullptr, *q = nullptr;
+    double *part = nullptr, *S = nullptr;
+    int nparts = 256;
+  };
+  MgState* mg = nullptr;
   std::vector<hipEvent_t> event_pool;
   std::vector<Mark> pending;
   size_t pool_used = 0;
@@ -120,6 +129,7 @@ struct aq_flip {
 
   ~aq_flip() {
     if (stream) hipStreamSynchronize(stream);
+    delete mg;
     for (StepGraph& g : graphs) {
       hipGraphExecDestroy(g.exec);
       hipGraphDestroy(g.graph);
@@ -425,6 +435,80 @@ aq_status run_stage(aq_flip* h, int st) {
   return AQ_OK;
 }
 
+// ---- multigrid-preconditioned CG ---------------------------------------------
+aq_status mg_setup(aq_flip* h) {
+  if (h->mg) return AQ_OK;
+  auto* M = new aq_flip::MgState;
+  h->mg = M;
+  const Geom& G = h->G;
+  int X = G.X, Y = G.Y, Z = G.Z;
+  for (;;) {  // halve every axis down to a few cells
+    MgLevel L{X, Y, Z, (long long)X * Y * Z, nullptr};
+    AQ_TRY(h->allocT(&L.fluid, (size_t)L.n));
+    float *f, *u, *t;
+    AQ_TRY(h->allocT(&f, (size_t)L.n));
+    AQ_TRY(h->allocT(&u, (size_t)L.n));
+    AQ_TRY(h->allocT(&t, (size_t)L.n));
+    M->H.levels.push_back(L);
+    M->H.f.push_back(f);
+    M->H.u.push_back(u);
+    M->H.t.push_back(t);
+    if (std::max(X, std::max(Y, Z)) <= 4 || M->H.levels.size() >= 12) break;
+    X = (X + 1) / 2, Y = (Y + 1) / 2, Z = (Z + 1) / 2;
+  }
+  const size_t n0 = (size_t)M->H.levels[0].n;
+  for (double** v : {&M->b, &M->x, &M->r, &M->z, &M->d, &M->q}) AQ_TRY(h->allocT(v, n0));
+  AQ_TRY(h->allocT(&M->part, (size_t)M->nparts * 2));
+  AQ_TRY(h->allocT(&M->S, 8));
+  return AQ_OK;
+}
+
+// Solve A p = b to an FP64 relative residual <= tol with MG-preconditioned CG in FP64,
+// starting from the current pressure (or zero), and store the result as FP32.
+aq_status mgpcg_solve(aq_flip* h, double tol, int max_iters, int check_every, bool from_current,
+                      int* iters, double* rel) {
+  AQ_TRY(mg_setup(h));
+  aq_flip::MgState& M = *h->mg;
+  const MgLevel& L0 = M.H.levels[0];
+  const long long n = L0.n;
+  hipStream_t s = h->stream;
+  mg_gather(h->G, h->gr.content, h->gr.rhs, h->gr.p[h->pb], from_current, L0, M.b, M.x, s);
+  for (size_t l = 1; l < M.H.levels.size(); ++l) mg_coarsen_flags(M.H.levels[l - 1], M.H.levels[l], s);
+  cg_residual(L0, M.b, M.x, M.r, s);
+  cg_dot2(n, M.b, M.b, nullptr, nullptr, M.part, M.nparts, M.S, 6, s);   // S[6] = b.b
+  mg_vcycle(M.H, M.r, M.z, s);
+  cg_copy(n, M.z, M.d, s);
+  cg_dot2(n, M.r, M.z, M.r, M.r, M.part, M.nparts, M.S, 0, s);           // S[0] = r.z, S[1] = r.r
+  double hs[8] = {0};
+  HIP_TRY(hipMemcpyAsync(hs, M.S, sizeof(hs), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const double bb = hs[6];
+  auto relres = [&](double rr) { return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr); };
+  double res = relres(hs[1]);
+  int it = 0;
+  while (!(res <= tol) && it < max_iters && std::isfinite(res)) {
+    cg_apply(L0, M.d, M.q, s);
+    cg_dot2(n, M.d, M.q, nullptr, nullptr, M.part, M.nparts, M.S, 2, s);  // S[2] = d.q
+    cg_update_xr(n, M.S, M.d, M.q, M.x, M.r, s);
+    mg_vcycle(M.H, M.r, M.z, s);
+    cg_dot2(n, M.r, M.z, M.r, M.r, M.part, M.nparts, M.S, 4, s);          // S[4] = r.z, S[5] = r.r
+    cg_update_d(n, M.S, M.z, M.d, s);
+    cg_shift(M.S, s);
+    ++it;
+    if (it % check_every == 0 || it == max_iters) {
+      HIP_TRY(hipMemcpyAsync(hs, M.S, sizeof(hs), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      res = relres(hs[1]);
+    }
+  }
+  mg_scatter(h->G, L0, M.x, h->gr.p[h->pb], s);
+  HIP_TRY(hipGetLastError());
+  h->air_clean[h->pb] = true;  // AIR cells written 0
+  *iters = it;
+  *rel = res;
+  return AQ_OK;
+}
+
 aq_status residual_now(aq_flip* h, double* rel) {
   launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
   HIP_TRY(hipGetLastError());
@@ -483,6 +567,7 @@ aq_status pressure_stage(aq_flip* h) {
   int sweeps = 0, total = 0;
   double res = 0, guess = 1.0;
   bool stalled = false, fallback = false;
+  bool mg_converged = false;  // MGPCG: the CG residual reached tol
   if (h->solver.mode == AQ_SOLVE_TOL_NN) {
     if (!h->net) return fail(AQ_EINVAL, "AQ_SOLVE_TOL_NN needs a U-Net");
     AQ_TRY(aq_flip_nn_guess(h, h->net));
@@ -507,6 +592,16 @@ aq_status pressure_stage(aq_flip* h) {
       st.nn_corrections += 1;
     }
   }
+  if (h->solver.mode == AQ_SOLVE_MGPCG) {
+    const bool from_current = h->cfg.pressure_init == AQ_PRESSURE_PREVIOUS;
+    int it = 0;
+    double cg = 0;
+    AQ_TRY(mgpcg_solve(h, h->solver.tol, h->solver.max_iters, h->solver.check_every, from_current, &it, &cg));
+    AQ_TRY(residual_now(h, &res));  // of the stored FP32 field (reported)
+    mg_converged = cg <= h->solver.tol;
+    total = it;
+    guess = 1.0;
+  }
   if (h->solver.mode == AQ_SOLVE_TOL_ZERO || fallback) {
     zero_pressure(h);
     AQ_TRY(sweep_to_tol(h, std::max(0, h->solver.max_iters), true, &sweeps, &res, &stalled));
@@ -517,7 +612,8 @@ aq_status pressure_stage(aq_flip* h) {
   st.solves += 1;
   st.sweeps += total;
   st.fallbacks += fallback ? 1 : 0;
-  st.capped += (res <= h->solver.tol) ? 0 : 1;
+  const bool reached = h->solver.mode == AQ_SOLVE_MGPCG ? mg_converged : res <= h->solver.tol;
+  st.capped += reached ? 0 : 1;
   st.last_sweeps = total;
   st.last_residual = res;
   st.last_guess_residual = guess;
@@ -820,7 +916,7 @@ extern "C" {
 
 aq_status aq_flip_set_solver(aq_flip* h, const aq_flip_solver* s, aq_unet* net) {
   if (!h || !s) return fail(AQ_EINVAL, "null argument");
-  if (s->mode < AQ_SOLVE_FIXED || s->mode > AQ_SOLVE_TOL_NN) return fail(AQ_EINVAL, "unknown solver mode");
+  if (s->mode < AQ_SOLVE_FIXED || s->mode > AQ_SOLVE_MGPCG) return fail(AQ_EINVAL, "unknown solver mode");
   if (s->mode != AQ_SOLVE_FIXED && (!(s->tol > 0) || s->max_iters < 0 || s->check_every < 1 ||
                                     s->stall_checks < 1 || s->nn_refresh < 0))
     return fail(AQ_EINVAL, "tolerance solves need tol > 0, max_iters >= 0, check_every >= 1, "
@@ -1403,6 +1499,19 @@ aq_status aq_flip_load(const char* path, aq_flip** out) {
   // on every AIR cell; the next sweeps write them fully (same values either way).
   h->air_clean[0] = h->air_clean[1] = false;
   *out = h;
+  return AQ_OK;
+}
+
+aq_status aq_flip_solve_pressure_mgpcg(aq_flip* h, double tol, int32_t max_iters, int32_t check_every,
+                                       int32_t from_current, int32_t* iters_out, double* rel_out) {
+  if (!h || max_iters < 0 || check_every < 1 || !(tol > 0)) return fail(AQ_EINVAL, "bad argument");
+  int it = 0;
+  double cg = 0, rel = 0;
+  AQ_TRY(mgpcg_solve(h, tol, max_iters, check_every, from_current != 0, &it, &cg));
+  AQ_TRY(residual_now(h, &rel));
+  if (iters_out) *iters_out = it;
+  if (rel_out) *rel_out = rel;
+  if (!(cg <= tol)) return fail(AQ_ESTALL, "MG-PCG reached its iteration cap");
   return AQ_OK;
 }
 

@@ -170,6 +170,18 @@ class FlipHandle:
             check(st)
         return it.value, rel.value, st == _abi.AQ_OK
 
+    def solve_pressure_mgpcg(self, tol: float, max_iters: int = 200, check_every: int = 1,
+                             from_current: bool = False):
+        """Multigrid-preconditioned CG (FP64). Returns (iterations, FP64 residual of the
+        stored FP32 pressure, converged)."""
+        it = C.c_int32()
+        rel = C.c_double()
+        st = self.lib.aq_flip_solve_pressure_mgpcg(self.h, tol, max_iters, check_every,
+                                                   1 if from_current else 0, C.byref(it), C.byref(rel))
+        if st not in (_abi.AQ_OK, _abi.AQ_ESTALL):
+            check(st)
+        return it.value, rel.value, st == _abi.AQ_OK
+
     # ---- timing
     def set_timing(self, on: bool) -> None:
         check(self.lib.aq_flip_set_timing(self.h, 1 if on else 0))

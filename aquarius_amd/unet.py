@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import (AQ_SOLVE_FIXED, AQ_SOLVE_TOL_NN, AQ_SOLVE_TOL_ZERO, Solver, SolverStats,
+from ._abi import (AQ_SOLVE_FIXED, AQ_SOLVE_MGPCG, AQ_SOLVE_TOL_NN, AQ_SOLVE_TOL_ZERO, Solver, SolverStats,
                    UNetConfig, check, load)
 
 
@@ -122,7 +122,8 @@ def nn_correct(flip, net: UNet) -> None:
 def attach_solver(flip, mode: str, tol: float = 1e-4, max_iters: int = 100000,
                   check_every: int = 50, stall_checks: int = 20, net: UNet | None = None,
                   nn_refresh: int = 0) -> None:
-    modes = {"fixed": AQ_SOLVE_FIXED, "tol_zero": AQ_SOLVE_TOL_ZERO, "tol_nn": AQ_SOLVE_TOL_NN}
+    modes = {"fixed": AQ_SOLVE_FIXED, "tol_zero": AQ_SOLVE_TOL_ZERO, "tol_nn": AQ_SOLVE_TOL_NN,
+             "mgpcg": AQ_SOLVE_MGPCG}
     s = Solver(modes[mode], tol, max_iters, check_every, stall_checks, nn_refresh)
     check(load().aq_flip_set_solver(flip.h, C.byref(s), net.h if net is not None else None))
 

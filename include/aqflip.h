@@ -193,6 +193,19 @@ aq_status aq_flip_solve_pressure(aq_flip* h, double tol, int32_t max_iters,
                                  int32_t check_every, int32_t* iters_out,
                                  double* rel_out);
 
+/* The same system solved by multigrid-preconditioned conjugate gradients in FP64
+ * (one geometric V-cycle per iteration), the accuracy reference and second
+ * fallback beside Jacobi; the reference's counterpart is the legacy sparse path,
+ * CSR assembly (Fluid/Legacy/PressureEquation.cuh:26-211,239-328) solved by
+ * AMGX-preconditioned CG (Fluid/Legacy/SPD_Solver.h:102-160). Starts from the
+ * current pressure when from_current != 0, else from zero; iterates until the CG
+ * residual is <= tol (checked every check_every iterations) or max_iters; stores
+ * the result as FP32. rel_out is the FP64 relative residual of that stored FP32
+ * field. Returns AQ_ESTALL when the CG residual did not reach tol. */
+aq_status aq_flip_solve_pressure_mgpcg(aq_flip* h, double tol, int32_t max_iters,
+                                       int32_t check_every, int32_t from_current,
+                                       int32_t* iters_out, double* rel_out);
+
 /* Per-stage device timing with HIP events on the handle's stream. When enabled,
  * every stage of aq_flip_step records events; aq_flip_get_timing returns the
  * accumulated milliseconds and launch counts per AQ_STAGE_* (arrays of

@@ -88,7 +88,11 @@ aq_status aq_flip_nn_guess(aq_flip* flip, aq_unet* net);
  * relative to the residual equation), then p += d on FLUID cells. */
 aq_status aq_flip_nn_correct(aq_flip* flip, aq_unet* net);
 
-enum { AQ_SOLVE_FIXED = 0, AQ_SOLVE_TOL_ZERO = 1, AQ_SOLVE_TOL_NN = 2 };
+/* FIXED: cfg.pressure_iters Jacobi sweeps (the reference). TOL_ZERO: Jacobi from zero
+ * to tol. TOL_NN: Jacobi from the U-Net guess to tol with U-Net corrections.
+ * MGPCG: multigrid-preconditioned CG in FP64 to tol (aq_flip_solve_pressure_mgpcg;
+ * max_iters and check_every count CG iterations, the stats' sweeps field too). */
+enum { AQ_SOLVE_FIXED = 0, AQ_SOLVE_TOL_ZERO = 1, AQ_SOLVE_TOL_NN = 2, AQ_SOLVE_MGPCG = 3 };
 typedef struct aq_flip_solver {
   int32_t mode;          /* AQ_SOLVE_* */
   double tol;            /* target FP64 relative residual */
