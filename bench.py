@@ -253,6 +253,11 @@ def main() -> int:
                           "nn_corrected_converged": ok3,
                           "nn_corrected_seconds": round(time.perf_counter() - ts, 3),
                           "corrected_iter_ratio": round(it / max(it3, 1), 3), "nn_refresh": NN_REFRESH})
+        # the accuracy reference: MG-preconditioned CG (FP64) from zero on the same system
+        ts = time.perf_counter()
+        itm, relm, okm = h.solve_pressure_mgpcg(1e-9, max_iters=200)
+        solve.update({"mgpcg_iters": itm, "mgpcg_residual_of_fp32_field": relm, "mgpcg_converged": okm,
+                      "mgpcg_seconds": round(time.perf_counter() - ts, 4)})
     h.close()
 
     tol_mode = None

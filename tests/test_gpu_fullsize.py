@@ -71,3 +71,28 @@ def test_ball_drop_64_many_steps_conserve():
     pos, vel, frac = h.get_particles()
     assert np.isfinite(pos).all() and np.isfinite(vel).all()
     h.close()
+
+
+def test_scale_256_warm_start():
+    """The 256^3 ball drop (about 45M particles): seeding, steps, and the warm start
+    with corrections against plain Jacobi at 1e-3 (1e-4 lies below the FP32 storage
+    floor of the pressure at this size: even the FP64 solution rounded to FP32 has a
+    residual near 3e-4)."""
+    from aquarius_amd import warmstart
+    from aquarius_amd.unet import UNet
+    if not warmstart.WEIGHTS.exists():
+        pytest.skip("no trained weights")
+    cfg = configs.ball_drop_256()
+    h = FlipHandle(cfg)
+    assert h.particle_count > 40_000_000
+    h.steps(3)
+    st = h.stats()
+    assert st["particles"] == h.particle_count
+    net = UNet()
+    net.load_file(warmstart.WEIGHTS)
+    r = warmstart.compare_on(h, net, tol=1e-3, cap=200000, check_every=100, refresh=200)
+    print(f"  256^3 to 1e-3: plain {r['plain_iters']}, guess {r['nn_iters']}, corrected {r['corr_iters']} "
+          f"({r['corrections']} corrections)", flush=True)
+    assert r["plain_ok"] and r["corr_ok"]
+    assert 3 * r["corr_iters"] <= r["plain_iters"]
+    h.close()
