@@ -43,8 +43,9 @@ STAGES = {
     "project": 7,
     "extrapolate": 8,
     "diffusion": 9,
+    "pressure_diffusion": 10,  # paired pressure + diffusion sweeps (timing only)
 }
-STAGE_COUNT = 10
+STAGE_COUNT = 11
 
 
 class Volume(C.Structure):

@@ -153,6 +153,10 @@ void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t 
 // without FLUID skipped -- valid once both buffers agree on AIR cells.
 void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc, int full,
                       hipStream_t s);
+// One Jacobi sweep (p[pin] -> p[pin^1]) and one diffusion sweep (nvf[fin] ->
+// nvf[fin^1]) in one launch; fullJ / fullD as `full` above.
+void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,
+                             double inv_cc, int fullD, hipStream_t s);
 void launch_g2p(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv, int nf,
                 StepParams sp, hipStream_t s);
 void launch_advect(const Geom& G, Particles P, long long Np, Grid gr, int nv, StepParams sp,

@@ -75,6 +75,9 @@ struct aq_flip {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool overlap = true;
+  // With the fixed-sweep solver, diffusion sweep i runs in the same launch as
+  // pressure sweep i (k_jacobi_diffusion) instead of on the second stream.
+  bool fuse = true;
   std::vector<void*> allocs;
 
   Particles P[2]{};
@@ -103,6 +106,7 @@ struct aq_flip {
   struct Mark {
     int stage;
     hipEvent_t a, b;
+    int64_t passes;
   };
   // Captured steps: one executable graph per ping-pong state at step entry (odd
   // pass counts alternate buffers between steps), with the state it leaves.
@@ -168,12 +172,12 @@ struct aq_flip {
     open_event = next_event();
     if (open_event) hipEventRecord(open_event, stream);
   }
-  void end(int st) {
+  void end(int st, int64_t passes = 1) {
     if (!timing || !open_event) return;
     hipEvent_t e = next_event();
     if (!e) return;
     hipEventRecord(e, stream);
-    pending.push_back({st, open_event, e});
+    pending.push_back({st, open_event, e, passes});
     open_event = nullptr;
   }
   // Called after the stream has been synchronised.
@@ -182,7 +186,7 @@ struct aq_flip {
       float ms = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms, m.a, m.b));
       stage_ms[m.stage] += ms;
-      stage_calls[m.stage] += 1;
+      stage_calls[m.stage] += m.passes;
     }
     pending.clear();
     pool_used = 0;
@@ -395,24 +399,55 @@ void stage_extrapolate(aq_flip* h) {
   }
 }
 
-void diffusion_sweeps(aq_flip* h, hipStream_t s) {
-  for (int i = 0; i < h->cfg.diffusion_iters; ++i) {  // the first sweep also copies AIR cells across
+// Diffusion sweeps first..diffusion_iters-1 of the step.
+void diffusion_sweeps(aq_flip* h, hipStream_t s, int first = 0) {
+  for (int i = first; i < h->cfg.diffusion_iters; ++i) {  // the first sweep also copies AIR cells across
     launch_diffusion(h->G, h->gr, h->K, h->nf, h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, s);
     h->nf ^= 1;
   }
 }
 
+// Fixed-sweep pressure solve and the diffusion sweeps together: the first
+// min(pressure_iters, diffusion_iters) sweeps of each run pairwise in one launch,
+// the rest of the longer chain on its own. Same iterates as the two loops.
+void fused_pressure_diffusion(aq_flip* h) {
+  if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) zero_pressure(h);
+  const int n = std::min(h->cfg.pressure_iters, h->cfg.diffusion_iters);
+  h->begin(AQ_STAGE_PRESSURE_DIFFUSION);
+  for (int i = 0; i < n; ++i) {
+    const int out = h->pb ^ 1;
+    launch_jacobi_diffusion(h->G, h->gr, h->K, h->pb, h->air_clean[out] ? 0 : 1, h->nf, h->sp.lambda,
+                            h->sp.inv_cc, i == 0 ? 1 : 0, h->stream);
+    h->air_clean[out] = true;
+    h->pb = out;
+    h->nf ^= 1;
+  }
+  h->end(AQ_STAGE_PRESSURE_DIFFUSION, n);
+  h->begin(AQ_STAGE_PRESSURE);
+  jacobi_sweeps(h, h->cfg.pressure_iters - n);
+  h->end(AQ_STAGE_PRESSURE, h->cfg.pressure_iters - n);
+  h->begin(AQ_STAGE_DIFFUSION);
+  diffusion_sweeps(h, h->stream, n);
+  h->end(AQ_STAGE_DIFFUSION, h->cfg.diffusion_iters - n);
+}
+
 // inStep: the grid fractions were produced by this step's P2G, so with a single
 // channel they are exactly 1 (FLUID) or 0 (AIR) and every sweep is the identity;
 // Jacobi with >= 1 sweep has already zeroed AIR pressure.
-void stage_diffusion(aq_flip* h, bool inStep) {
-  if (h->cfg.diffusion_iters <= 0) return;
+// Returns the number of diffusion sweeps run.
+int stage_diffusion(aq_flip* h, bool inStep) {
+  if (h->cfg.diffusion_iters <= 0) return 0;
   if (!inStep || h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
-  if (inStep && h->K == 1) return;
+  if (inStep && h->K == 1) return 0;
   diffusion_sweeps(h, h->stream);
+  return h->cfg.diffusion_iters;
 }
 
 aq_status run_stage(aq_flip* h, int st) {
+  if (st == AQ_STAGE_PRESSURE_DIFFUSION)
+    return fail(AQ_EINVAL, "AQ_STAGE_PRESSURE_DIFFUSION runs only inside a step");
+  if (st < 0 || st >= AQ_STAGE_COUNT) return fail(AQ_EINVAL, "unknown stage");
+  int64_t passes = 1;
   h->begin(st);
   switch (st) {
     case AQ_STAGE_G2P: launch_g2p(h->G, h->P[h->cur], h->Np, h->K, h->gr, h->nv, h->nf, h->sp, h->stream); break;
@@ -424,14 +459,20 @@ aq_status run_stage(aq_flip* h, int st) {
     case AQ_STAGE_GRAVITY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 1, 0, h->stream); break;
     case AQ_STAGE_BOUNDARY: launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 0, 1, h->stream); break;
     case AQ_STAGE_DIVERGENCE: launch_divergence(h->G, h->gr, h->nv, h->B, h->stream); break;
-    case AQ_STAGE_PRESSURE: stage_pressure(h); break;
+    case AQ_STAGE_PRESSURE:
+      stage_pressure(h);
+      passes = h->cfg.pressure_iters;
+      break;
     case AQ_STAGE_PROJECT: launch_project(h->G, h->gr, h->nv, h->hv, h->pb, h->stream); break;
-    case AQ_STAGE_EXTRAPOLATE: stage_extrapolate(h); break;
-    case AQ_STAGE_DIFFUSION: stage_diffusion(h, false); break;
-    default: return fail(AQ_EINVAL, "unknown stage");
+    case AQ_STAGE_EXTRAPOLATE:
+      stage_extrapolate(h);
+      passes = 3;
+      break;
+    case AQ_STAGE_DIFFUSION: passes = stage_diffusion(h, false); break;
+    default: break;
   }
   HIP_TRY(hipGetLastError());
-  h->end(st);
+  h->end(st, passes);
   return AQ_OK;
 }
 
@@ -560,7 +601,7 @@ aq_status pressure_stage(aq_flip* h) {
     if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) zero_pressure(h);
     h->begin(AQ_STAGE_PRESSURE);  // the Jacobi sweeps only
     jacobi_sweeps(h, h->cfg.pressure_iters);
-    h->end(AQ_STAGE_PRESSURE);
+    h->end(AQ_STAGE_PRESSURE, h->cfg.pressure_iters);
     return AQ_OK;
   }
   h->begin(AQ_STAGE_PRESSURE);
@@ -608,7 +649,7 @@ aq_status pressure_stage(aq_flip* h) {
     total += sweeps;
     if (h->solver.mode == AQ_SOLVE_TOL_ZERO) guess = 1.0;
   }
-  h->end(AQ_STAGE_PRESSURE);
+  h->end(AQ_STAGE_PRESSURE, total);
   st.solves += 1;
   st.sweeps += total;
   st.fallbacks += fallback ? 1 : 0;
@@ -640,14 +681,18 @@ aq_status enqueue_step(aq_flip* h) {
   // Diffusion reads the fractions, content and active-block flags, all final after
   // the divergence pass, and writes only the fractions, which nothing before the
   // next step's G2P reads. (Stage timing keeps the reference's sequential order.)
-  const bool side = h->overlap && !h->timing && h->cfg.diffusion_iters > 0 && h->K > 1;
+  const bool fused = h->fuse && h->solver.mode == AQ_SOLVE_FIXED &&
+                     h->cfg.diffusion_iters > 0 && h->K > 1 && h->cfg.pressure_iters > 0;
+  const bool side = !fused && h->overlap && !h->timing && h->cfg.diffusion_iters > 0 && h->K > 1;
   if (side) {
     HIP_TRY(hipEventRecord(h->ev_fork, h->stream));
     HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
     diffusion_sweeps(h, h->side);
     HIP_TRY(hipEventRecord(h->ev_join, h->side));
   }
-  if (const aq_status ps = pressure_stage(h); ps != AQ_OK) {
+  if (fused) {
+    fused_pressure_diffusion(h);
+  } else if (const aq_status ps = pressure_stage(h); ps != AQ_OK) {
     if (side) hipStreamWaitEvent(h->stream, h->ev_join, 0);  // never leave the fork open
     return ps;
   }
@@ -656,14 +701,13 @@ aq_status enqueue_step(aq_flip* h) {
   h->end(AQ_STAGE_PROJECT);
   h->begin(AQ_STAGE_EXTRAPOLATE);
   stage_extrapolate(h);
-  h->end(AQ_STAGE_EXTRAPOLATE);
+  h->end(AQ_STAGE_EXTRAPOLATE, 3);
   if (side) {
     if (h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);
     HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));
-  } else {
+  } else if (!fused) {
     h->begin(AQ_STAGE_DIFFUSION);
-    stage_diffusion(h, true);
-    h->end(AQ_STAGE_DIFFUSION);
+    h->end(AQ_STAGE_DIFFUSION, stage_diffusion(h, true));
   }
   HIP_TRY(hipGetLastError());
   h->physicalTime += h->cfg.dt;
@@ -838,11 +882,19 @@ aq_status write_seeded(aq_flip* h, const SeedPlan& plan) {
   return AQ_OK;
 }
 
+// Switches for measurements: AQFLIP_NO_GRAPH=1 (eager launches), AQFLIP_NO_OVERLAP=1
+// (diffusion after the pressure solve on one stream), AQFLIP_NO_FUSE=1 (no fused
+// pressure + diffusion sweeps). Results are identical in every combination.
+void read_env(aq_flip* h) {
+  if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
+  if (const char* env = std::getenv("AQFLIP_NO_OVERLAP")) h->overlap = env[0] != '1';
+  if (const char* env = std::getenv("AQFLIP_NO_FUSE")) h->fuse = env[0] != '1';
+}
+
 aq_status create_common(const aq_flip_config* cfg, const float* pos, const float* vel,
                         const float* frac4, long long n, aq_flip** out, const SeedPlan* plan = nullptr) {
   aq_flip* h = new aq_flip;
-  if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
-  if (const char* env = std::getenv("AQFLIP_NO_OVERLAP")) h->overlap = env[0] != '1';
+  read_env(h);
   aq_status st = setup(h, cfg, n);
   if (plan) {  // seeded on the device: positions are clamped into the domain by construction
     h->K = channels_for(*cfg, nullptr, 0);
@@ -1464,8 +1516,7 @@ aq_status aq_flip_load(const char* path, aq_flip** out) {
   if (validate(&cfg) != AQ_OK) return bail(AQ_EINVAL, std::string("snapshot configuration: ") + aq_last_error());
   if (np <= 0 || np > (1ll << 31) - 1 || K < 1 || K > 4) return bail(AQ_EINVAL, "corrupt snapshot state");
   aq_flip* h = new aq_flip;
-  if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
-  if (const char* env = std::getenv("AQFLIP_NO_OVERLAP")) h->overlap = env[0] != '1';
+  read_env(h);
   aq_status st = setup(h, &cfg, np);
   if (st != AQ_OK) {
     delete h;

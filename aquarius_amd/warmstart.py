@@ -1,9 +1,12 @@
 """Learned pressure warm start: training data, training and evaluation.
 
-Training pairs come from the plain solver: randomized ball-drop scenes (1-3
+Training pairs come from the simulation: randomized ball-drop scenes (1-3
 spheres of radius U[0.1, 0.25] above a pool of depth U[0.1, 0.4], seeded) are
 stepped with the fixed-sweep solver, and at chosen steps the step's pressure
-system is re-solved from zero with plain FP32 Jacobi to a tight FP64 residual;
+system is solved to convergence -- by default with plain FP32 Jacobi to a tight
+FP64 residual, or with the MG-preconditioned CG solver (`--mgpcg`, FP64, to a 1e-10
+CG residual); a pair is accepted only if the FP64 residual of its stored FP32
+pressure is <= 1e-4;
 (right-hand side, FLUID mask, converged pressure) is captured on the device.
 
     python -m aquarius_amd.warmstart train --pairs 4000 --out aquarius_amd/weights/unet_warmstart_64.npz
@@ -52,7 +55,7 @@ def converge(h: FlipHandle, tol: float, cap: int, check_every: int = 100, start=
 
 def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int = 4,
              max_step: int = 60, tol: float = 1e-5, accept: float = 1e-4, cap: int = 60000,
-             log=print):
+             log=print, solver: str = "jacobi"):
     """Fill the network's device dataset with `pairs` converged systems. Within a
     scene each system is solved starting from the previous stop's converged pressure
     (the fixed point does not depend on the start, only the sweep count does)."""
@@ -70,18 +73,21 @@ def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int =
         for s in stops:
             h.steps(s - done)
             done = s
-            it, rel, _ = converge(h, tol, cap, start=prev)
+            if solver == "mgpcg":
+                it, rel, _ = h.solve_pressure_mgpcg(1e-10, max_iters=300)
+            else:
+                it, rel, _ = converge(h, tol, cap, start=prev)
             sweeps += it
             if rel <= accept and net.dataset_size < pairs:
                 net.add_sample(h)
-                prev = h.get_field("pressure")
+                prev = h.get_field("pressure") if solver != "mgpcg" else None
         h.close()
         scene += 1
         if scene % 50 == 0:
-            log(f"  scenes {scene}, pairs {net.dataset_size}/{pairs}, mean sweeps "
+            log(f"  scenes {scene}, pairs {net.dataset_size}/{pairs}, mean {solver} iterations "
                 f"{sweeps / max(1, net.dataset_size):.0f}, {time.time() - t0:.0f} s")
-    return dict(scenes=scene, pairs=net.dataset_size, seconds=time.time() - t0,
-                mean_sweeps=sweeps / max(1, net.dataset_size))
+    return dict(scenes=scene, pairs=net.dataset_size, seconds=time.time() - t0, solver=solver,
+                mean_iterations=sweeps / max(1, net.dataset_size))
 
 
 def solve_with_corrections(h: FlipHandle, net: UNet, tol: float, cap: int, refresh: int,
@@ -175,6 +181,7 @@ def main(argv=None) -> int:
     t.add_argument("--eval-scenes", type=int, default=8)
     t.add_argument("--tol", type=float, default=1e-5)
     t.add_argument("--eval-128", action="store_true")
+    t.add_argument("--mgpcg", action="store_true", help="converge the training systems with MG-PCG")
     c = sub.add_parser("compare")
     c.add_argument("--weights", type=Path, default=WEIGHTS)
     c.add_argument("--n", type=int, default=64)
@@ -187,7 +194,7 @@ def main(argv=None) -> int:
         net = UNet(lr=a.lr, seed=a.seed + 1)
         total = a.pairs + a.held_out
         print(f"[warmstart] generating {total} pairs at {a.n}^3", flush=True)
-        gen = generate(net, total, n=a.n, seed=a.seed, tol=a.tol)
+        gen = generate(net, total, n=a.n, seed=a.seed, tol=a.tol, solver="mgpcg" if a.mgpcg else "jacobi")
         print(f"[warmstart] data: {json.dumps(gen)}", flush=True)
         train_n = a.pairs
         l0 = net.eval(train_n, a.held_out)

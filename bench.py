@@ -56,13 +56,9 @@ def algorithmic_bytes(cfg, n_particles: int, K: int) -> dict:
         "project": C * (1 + 4 + 4 + 12 + 12 + 1),
         "extrapolate": C * (12 + 1 + 12 + 1),  # per pass
         "diffusion": C * (1 + 8 * K),  # per sweep
+        # one pressure sweep and one diffusion sweep in the same launch
+        "pressure_diffusion": C * (4 + 4 + 1 + 4) + C * (1 + 8 * K),
     }
-
-
-def launches_per_step(cfg, K: int) -> dict:
-    return {"g2p": 1, "transfer_to_grid": 1, "gravity": 1, "divergence": 1,
-            "pressure": cfg.pressure_iters, "project": 1, "extrapolate": 3,
-            "diffusion": cfg.diffusion_iters if K > 1 else 0}
 
 
 # Kernels of each stage, as rocprofv3 names them (template arguments stripped).
@@ -77,6 +73,7 @@ STAGE_KERNELS = {
     "project": ["aqf::k_project"],
     "extrapolate": ["aqf::k_extrapolate"],
     "diffusion": ["aqf::k_diffusion"],
+    "pressure_diffusion": ["aqf::k_jacobi_diffusion"],
 }
 
 
@@ -197,7 +194,9 @@ def main() -> int:
     ms_per_step = 1000.0 * wall / args.steps
 
     # Profiled pass: the same K steps launched eagerly with HIP events recorded on
-    # the library's stream around every stage (read back once, at the end).
+    # the library's stream around every stage (read back once, at the end). The
+    # library reports each stage's passes (sweeps, extrapolation passes, or runs),
+    # one kernel launch per pass for the sweep stages.
     h.reset_timing()
     h.set_timing(True)
     tp0 = time.perf_counter()
@@ -208,15 +207,14 @@ def main() -> int:
     eager_ms_per_step = 1000.0 * (tp1 - tp0) / args.steps
 
     nbytes = algorithmic_bytes(cfg, n, K)
-    per_step = launches_per_step(cfg, K)
     kernels = {}
-    for name, (ms, calls) in timing.items():
-        if name not in nbytes or calls == 0 or per_step[name] == 0:
+    for name, (ms, launches) in timing.items():
+        if name not in nbytes or launches == 0:
             continue
-        launches = per_step[name] * args.steps
         avg_us = 1000.0 * ms / launches
         gbs = nbytes[name] / (avg_us * 1e-6) / 1e9
-        kernels[name] = {"ms_per_step": round(ms / args.steps, 4), "launches_per_step": per_step[name],
+        kernels[name] = {"ms_per_step": round(ms / args.steps, 4),
+                         "launches_per_step": round(launches / args.steps, 3),
                          "avg_launch_us": round(avg_us, 3), "bytes_per_launch": nbytes[name],
                          "achieved_GBs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])

@@ -109,7 +109,10 @@ enum {
   AQ_STAGE_PROJECT = 7,          /* updateVelocityWithPressure  Fluid_3D_kernels.cu:118-207 */
   AQ_STAGE_EXTRAPOLATE = 8,      /* extrapolateVelocity (3 passes)  Fluid_3D_common.cu:65-82 */
   AQ_STAGE_DIFFUSION = 9,        /* solveDiffusionJacobi     Fluid_3D_common.cu:85-91 */
-  AQ_STAGE_COUNT = 10
+  /* Pressure and diffusion sweeps paired in one launch inside a fixed-sweep step (the
+   * two chains are independent); a timing bucket only, not runnable on its own. */
+  AQ_STAGE_PRESSURE_DIFFUSION = 10,
+  AQ_STAGE_COUNT = 11
 };
 
 typedef struct aq_flip aq_flip; /* opaque handle */
@@ -207,9 +210,10 @@ aq_status aq_flip_solve_pressure_mgpcg(aq_flip* h, double tol, int32_t max_iters
                                        int32_t* iters_out, double* rel_out);
 
 /* Per-stage device timing with HIP events on the handle's stream. When enabled,
- * every stage of aq_flip_step records events; aq_flip_get_timing returns the
- * accumulated milliseconds and launch counts per AQ_STAGE_* (arrays of
- * AQ_STAGE_COUNT) since the last reset. */
+ * every stage of aq_flip_step records events; aq_flip_get_timing returns, per
+ * AQ_STAGE_* (arrays of AQ_STAGE_COUNT) since the last reset, the accumulated
+ * milliseconds and the number of passes: sweeps for the pressure, diffusion and
+ * paired stages, 3 per extrapolation, 1 per run of any other stage. */
 aq_status aq_flip_set_timing(aq_flip* h, int32_t enable);
 aq_status aq_flip_get_timing(aq_flip* h, double* stage_ms, int64_t* stage_calls);
 aq_status aq_flip_reset_timing(aq_flip* h);

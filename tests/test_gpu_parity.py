@@ -218,13 +218,15 @@ def test_python_fluid_surface():
     f.draw()
 
 
-def test_full_steps_match_oracle_multiphase_24():
+@pytest.mark.parametrize("pressure_iters,diffusion_iters", [(40, 10), (12, 30)])
+def test_full_steps_match_oracle_multiphase_24(pressure_iters, diffusion_iters):
     """Whole steps of a two-phase scene through the captured step graph, whose
-    diffusion sweeps run on a second stream concurrently with the pressure solve."""
-    cfg = configs.make_config(24, dt=0.01, pressure_iters=40, diffusion_iters=10, radius=0.15,
-                              sphere_phase=1, phase_count=2, seed=21)
+    diffusion sweeps share launches with the pressure sweeps (the longer chain
+    finishing on its own: pressure in the first case, diffusion in the second)."""
+    cfg = configs.make_config(24, dt=0.01, pressure_iters=pressure_iters, diffusion_iters=diffusion_iters,
+                              radius=0.15, sphere_phase=1, phase_count=2, seed=21)
     gpu, orc = pair(cfg)
-    log("[full steps multiphase 24] 4 steps")
+    log(f"[full steps multiphase 24 J{pressure_iters} D{diffusion_iters}] 4 steps")
     gpu.steps(4)
     orc.step(4)
     check_particles(gpu, orc, "4 steps ")
@@ -235,16 +237,56 @@ def test_full_steps_match_oracle_multiphase_24():
 
 
 def test_concurrent_diffusion_equals_sequential_128(monkeypatch):
+    """Fused pressure + diffusion launches, diffusion on a second stream, and the
+    reference's sequential order give the same bits at 128^3."""
     cfg = configs.multiphase_128()
     pos, frac = ob.seed(cfg)
     out = {}
-    for flag in ("1", "0"):
-        monkeypatch.setenv("AQFLIP_NO_OVERLAP", flag)
+    for mode, (no_fuse, no_overlap) in {"fused": ("0", "0"), "overlap": ("1", "0"),
+                                        "sequential": ("1", "1")}.items():
+        monkeypatch.setenv("AQFLIP_NO_FUSE", no_fuse)
+        monkeypatch.setenv("AQFLIP_NO_OVERLAP", no_overlap)
         h = FlipHandle(cfg, pos=pos, frac4=frac)
-        log(f"[128 no_overlap={flag}] 3 steps")
+        log(f"[128 {mode}] 3 steps")
         h.steps(3)
-        out[flag] = {f: h.get_field(f) for f in ("pressure", "new_velocity", "new_volume_fractions")}
-        out[flag]["pos"] = h.get_particles()[0]
+        out[mode] = {f: h.get_field(f) for f in ("pressure", "new_velocity", "new_volume_fractions")}
+        out[mode]["pos"] = h.get_particles()[0]
         h.close()
-    for k in out["0"]:
-        assert np.array_equal(out["0"][k].view(np.uint32), out["1"][k].view(np.uint32)), k
+    for mode in ("fused", "overlap"):
+        for k in out[mode]:
+            assert np.array_equal(out[mode][k].view(np.uint32), out["sequential"][k].view(np.uint32)), (mode, k)
+
+
+def test_stage_timing_counts_and_matches_graph(monkeypatch):
+    """The profiled (eager, event-timed) step runs the same paired sweeps as the
+    captured step, reports one pass per sweep, and gives the same bits."""
+    cfg = configs.make_config(24, dt=0.01, pressure_iters=40, diffusion_iters=10, radius=0.15,
+                              sphere_phase=1, phase_count=2, seed=21)
+    pos, frac = ob.seed(cfg)
+    out = {}
+    for mode in ("graph", "timed"):
+        h = FlipHandle(cfg, pos=pos, frac4=frac)
+        log(f"[timing {mode}] 2 steps")
+        if mode == "timed":
+            h.set_timing(True)
+        h.steps(2)
+        if mode == "timed":
+            t = h.timing()
+            assert t["pressure_diffusion"][1] == 2 * 10
+            assert t["pressure"][1] == 2 * 30
+            assert t["diffusion"][1] == 0
+            assert t["extrapolate"][1] == 2 * 3
+            assert t["g2p"][1] == 2 and t["transfer_to_grid"][1] == 2
+            assert all(ms >= 0 for ms, _ in t.values())
+        out[mode] = {f: h.get_field(f) for f in ("pressure", "new_velocity", "new_volume_fractions")}
+        out[mode]["pos"] = h.get_particles()[0]
+        h.close()
+    for k in out["graph"]:
+        assert np.array_equal(out["graph"][k].view(np.uint32), out["timed"][k].view(np.uint32)), k
+    monkeypatch.setenv("AQFLIP_NO_FUSE", "1")
+    h = FlipHandle(cfg, pos=pos, frac4=frac)
+    h.set_timing(True)
+    h.steps(1)
+    t = h.timing()
+    h.close()
+    assert t["pressure_diffusion"][1] == 0 and t["pressure"][1] == 40 and t["diffusion"][1] == 10
