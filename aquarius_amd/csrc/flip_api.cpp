@@ -302,6 +302,8 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   AQ_TRY(h->allocT(&gr.div, nc));
   AQ_TRY(h->allocT(&gr.rhs, nc));
   AQ_TRY(h->allocT(&gr.density, nc));
+  // One flag per 256-cell block plus one slot that stays 0, so the sweep kernels'
+  // pairs of blocks never read past the array.
   AQ_TRY(h->allocT(&gr.active, (size_t)(G.ncells_hash + 255) / 256 + 1));
   for (int a = 0; a < 4; ++a) {
     AQ_TRY(h->allocT(&gr.vf[a], nc));

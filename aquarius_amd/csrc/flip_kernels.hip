@@ -688,53 +688,85 @@ void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
                      gr.p[pb]);
 }
 
-// jacobiImpl (Fluid/Fluid_3D_kernels.cu:460-505), read-old/write-new: AIR cells 0,
-// otherwise (sum of the six clamp-to-edge neighbours + rhs) / 6, summed in the
-// reference order x+1, x-1, y+1, y-1, z+1, z-1, rhs.
-// AIR cells are recognised from the right-hand side's reserved bit pattern.
-// All seven loads are issued before the AIR test, so a cell costs one memory
-// round trip; an AIR cell's (unused) sum is discarded. `full` only documents the
-// caller's state: AIR cells are always written (0), which is what a full sweep
-// needs and what a non-full sweep's buffer already holds.
-__device__ __forceinline__ void jacobi_cell(const Geom& G, const float* __restrict__ rhs,
-                                            const float* __restrict__ pin, float* __restrict__ pout,
-                                            int n, int full) {
+// Cells per thread of the pressure sweep kernels: a thread issues the loads of all
+// its cells before it uses any, and a launch needs 1/kSweepCells of the
+// workgroups. At 128^3 two cells per thread took a pressure sweep from 5.30 to
+// 4.83 us and a paired sweep from 8.75 to 8.31 us; four were slower (6.1 us).
+constexpr int kSweepCells = 2;
+constexpr int kSweepSpan = kSweepCells * kBlock;  // cells per workgroup
+
+// Cell c (0..kSweepCells-1) of this thread, its grid index and its six
+// clamp-to-edge neighbours in the reference's order x+1, x-1, y+1, y-1, z+1, z-1.
+// Threads past the last cell get cell 0 (in bounds) and ok = false.
+struct SweepCell {
+  long long gi, nb[6];
+  bool ok;
+};
+__device__ __forceinline__ SweepCell sweep_cell(const Geom& G, int lb, int c) {
+  SweepCell q;
+  const int n = lb * kSweepSpan + c * kBlock + (int)threadIdx.x;
+  q.ok = n < G.ncells_hash;
   int x, y, z;
-  cell_of(G, n, x, y, z);
-  const long long gi = G.g(x, y, z);
-  const long long xp = x < G.X - 1 ? G.sx : 0, xm = x > 0 ? G.sx : 0;
-  const long long yp = y < G.Y - 1 ? G.EZ : 0, ym = y > 0 ? G.EZ : 0;
-  const long long zp = z < G.Z - 1 ? 1 : 0, zm = z > 0 ? 1 : 0;
-  const float b = rhs[gi];
-  const float v0 = pin[gi + xp], v1 = pin[gi - xm], v2 = pin[gi + yp];
-  const float v3 = pin[gi - ym], v4 = pin[gi + zp], v5 = pin[gi - zm];
-  // Empty asm that consumes the seven values: keeps the compiler from moving the
-  // neighbour loads behind the AIR test (a second round trip).
-  asm volatile("" ::"v"(b), "v"(v0), "v"(v1), "v"(v2), "v"(v3), "v"(v4), "v"(v5));
+  cell_of(G, q.ok ? n : 0, x, y, z);
+  q.gi = G.g(x, y, z);
+  q.nb[0] = q.gi + (x < G.X - 1 ? G.sx : 0);
+  q.nb[1] = q.gi - (x > 0 ? G.sx : 0);
+  q.nb[2] = q.gi + (y < G.Y - 1 ? (long long)G.EZ : 0);
+  q.nb[3] = q.gi - (y > 0 ? (long long)G.EZ : 0);
+  q.nb[4] = q.gi + (z < G.Z - 1 ? 1 : 0);
+  q.nb[5] = q.gi - (z > 0 ? 1 : 0);
+  return q;
+}
+// A workgroup's cells hold FLUID if any of the 256-cell blocks they span does
+// (flags of k_divergence; the slot past the last block is always 0).
+__device__ __forceinline__ bool sweep_active(const uint32_t* __restrict__ active, int lb) {
+  bool any = false;
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c) any = any || active[lb * kSweepCells + c] != 0;
+  return any;
+}
+// jacobiImpl (Fluid/Fluid_3D_kernels.cu:460-505) for one cell whose loads are done:
+// AIR cells 0, otherwise (x+1 + x-1 + y+1 + y-1 + z+1 + z-1 + rhs) / 6 summed in
+// that order. AIR cells are recognised from the right-hand side's reserved bit
+// pattern.
+__device__ __forceinline__ float jacobi_value(float b, const float v[6]) {
   float s = 0.f;
-  s += v0;
-  s += v1;
-  s += v2;
-  s += v3;
-  s += v4;
-  s += v5;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s += v[k];
   s += b;
-  // AIR cells get 0; without `full` the buffer already holds 0 there, so storing
-  // unconditionally keeps every load on the path of the store.
-  const bool air = __float_as_uint(b) == kAirRhsBits;
-  pout[gi] = air ? 0.f : div_exact(s, 1.0 / 6.0);
+  return __float_as_uint(b) == kAirRhsBits ? 0.f : div_exact(s, 1.0 / 6.0);
 }
 
+// One Jacobi sweep, read-old/write-new. All loads of a thread's cells are issued
+// before any is used (an empty `asm volatile` consumes them, so the compiler cannot
+// sink loads behind a branch), so a cell costs one memory round trip. AIR cells are
+// always written (0): what a full sweep needs and what a non-full sweep's buffer
+// already holds; without `full`, workgroups without FLUID are skipped.
 __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
                          float* __restrict__ pout, const uint32_t* __restrict__ active, int full) {
-  if (!full && !active[xcd_block()]) return;
-  const int n = gtid();
-  if (n >= G.ncells_hash) return;
-  jacobi_cell(G, rhs, pin, pout, n, full);
+  const int lb = xcd_block();
+  if (!full && !sweep_active(active, lb)) return;
+  SweepCell q[kSweepCells];
+  float b[kSweepCells], v[kSweepCells][6];
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c) {
+    q[c] = sweep_cell(G, lb, c);
+    b[c] = rhs[q[c].gi];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[c][k] = pin[q[c].nb[k]];
+  }
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c)
+    asm volatile("" ::"v"(b[c]), "v"(v[c][0]), "v"(v[c][1]), "v"(v[c][2]), "v"(v[c][3]), "v"(v[c][4]),
+                 "v"(v[c][5]));
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c)
+    if (q[c].ok) pout[q[c].gi] = jacobi_value(b[c], v[c]);
 }
 void launch_jacobi(const Geom& G, Grid gr, int pin, int full, hipStream_t s) {
-  hipLaunchKernelGGL(k_jacobi, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr.rhs,
-                     gr.p[pin], gr.p[pin ^ 1], gr.active, full);
+  const int nb = (G.ncells_hash + kSweepSpan - 1) / kSweepSpan;
+  hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin], gr.p[pin ^ 1],
+                     gr.active, full);
 }
 
 // diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
@@ -943,81 +975,80 @@ void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, doub
 }
 
 // One Jacobi sweep and one diffusion sweep in a single launch. The two chains touch
-// disjoint fields (pressure, and the volume fractions), so a thread can run both
-// for its cell: both sweeps are the same arithmetic as jacobi_cell and
-// diffusion_cell, and the shared cell and neighbour indices are computed once.
-// Every load of both is issued before either is consumed, so the cell still costs
-// one memory round trip. fullJ and fullD mean what `full` means for each sweep.
+// disjoint fields (pressure, and the volume fractions), so a thread runs both for
+// its cells with the arithmetic of jacobi_value and diffusion_cell, computing the
+// shared cell and neighbour indices once. Every load of both is issued before any
+// is used, so the cells still cost one memory round trip. fullJ and fullD mean what
+// `full` means for each sweep.
 template <int K>
 __global__ void k_jacobi_diffusion(Geom G, Grid gr, int pin, int fullJ, int fin, float lambda,
                                    double inv_cc, int fullD) {
-  const bool act = gr.active[xcd_block()] != 0;
+  const int lb = xcd_block();
+  const bool act = sweep_active(gr.active, lb);
   const bool doJ = fullJ || act, doD = fullD || act;
   if (!doJ && !doD) return;
-  const int n = gtid();
-  if (n >= G.ncells_hash) return;
-  int x, y, z;
-  cell_of(G, n, x, y, z);
-  const long long gi = G.g(x, y, z);
-  const long long nb[6] = {gi + (x < G.X - 1 ? G.sx : 0), gi - (x > 0 ? G.sx : 0),
-                           gi + (y < G.Y - 1 ? (long long)G.EZ : 0),
-                           gi - (y > 0 ? (long long)G.EZ : 0), gi + (z < G.Z - 1 ? 1 : 0),
-                           gi - (z > 0 ? 1 : 0)};
   const float* __restrict__ P = gr.p[pin];
-  const float b = gr.rhs[gi];
-  float pv[6];
+  SweepCell q[kSweepCells];
+  float b[kSweepCells], pv[kSweepCells][6], v[kSweepCells][7][K];
+  uint8_t cc[kSweepCells];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) pv[k] = P[nb[k]];
-  const uint8_t c = gr.content[gi];
-  float v[7][K];
+  for (int c = 0; c < kSweepCells; ++c) {
+    q[c] = sweep_cell(G, lb, c);
+    b[c] = gr.rhs[q[c].gi];
 #pragma unroll
-  for (int k = 0; k < 6; ++k)
+    for (int k = 0; k < 6; ++k) pv[c][k] = P[q[c].nb[k]];
+    cc[c] = gr.content[q[c].gi];
 #pragma unroll
-    for (int a = 0; a < K; ++a) v[k][a] = gr.nvf[fin][a][nb[k]];
+    for (int k = 0; k < 6; ++k)
 #pragma unroll
-  for (int a = 0; a < K; ++a) v[6][a] = gr.nvf[fin][a][gi];
-  asm volatile("" ::"v"(b), "v"(pv[0]), "v"(pv[1]), "v"(pv[2]), "v"(pv[3]), "v"(pv[4]), "v"(pv[5]));
+      for (int a = 0; a < K; ++a) v[c][k][a] = gr.nvf[fin][a][q[c].nb[k]];
 #pragma unroll
-  for (int k = 0; k < 7; ++k)
-#pragma unroll
-    for (int a = 0; a < K; ++a) asm volatile("" ::"v"(v[k][a]));
-  if (doJ) {  // jacobi_cell: x+1, x-1, y+1, y-1, z+1, z-1, rhs
-    float s = 0.f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) s += pv[k];
-    s += b;
-    const bool air = __float_as_uint(b) == kAirRhsBits;
-    gr.p[pin ^ 1][gi] = air ? 0.f : div_exact(s, 1.0 / 6.0);
+    for (int a = 0; a < K; ++a) v[c][6][a] = gr.nvf[fin][a][q[c].gi];
   }
-  if (!doD) return;
-  if (c == AIR) {  // diffusion_cell
-    if (fullD) {
 #pragma unroll
-      for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = v[6][a];
+  for (int c = 0; c < kSweepCells; ++c) {
+    asm volatile("" ::"v"(b[c]), "v"(pv[c][0]), "v"(pv[c][1]), "v"(pv[c][2]), "v"(pv[c][3]),
+                 "v"(pv[c][4]), "v"(pv[c][5]));
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+#pragma unroll
+      for (int a = 0; a < K; ++a) asm volatile("" ::"v"(v[c][k][a]));
+  }
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c) {
+    if (!q[c].ok) continue;
+    const long long gi = q[c].gi;
+    if (doJ) gr.p[pin ^ 1][gi] = jacobi_value(b[c], pv[c]);
+    if (!doD) continue;
+    if (cc[c] == AIR) {  // diffusion_cell: AIR cells keep their value
+      if (fullD) {
+#pragma unroll
+        for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = v[c][6][a];
+      }
+      continue;
     }
-    return;
+    float r[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) r[a] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+#pragma unroll
+      for (int a = 0; a < K; ++a) r[a] += v[c][k][a] * lambda;
+    }
+#pragma unroll
+    for (int a = 0; a < K; ++a) r[a] += v[c][6][a];
+#pragma unroll
+    for (int a = 0; a < K; ++a) r[a] = div_exact(r[a], inv_cc);  // r / (1 + 2 lambda)
+    float sum = r[0];
+#pragma unroll
+    for (int a = 1; a < K; ++a) sum += r[a];
+#pragma unroll
+    for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = r[a] / sum;
   }
-  float r[K];
-#pragma unroll
-  for (int a = 0; a < K; ++a) r[a] = 0.f;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-#pragma unroll
-    for (int a = 0; a < K; ++a) r[a] += v[k][a] * lambda;
-  }
-#pragma unroll
-  for (int a = 0; a < K; ++a) r[a] += v[6][a];
-#pragma unroll
-  for (int a = 0; a < K; ++a) r[a] = div_exact(r[a], inv_cc);
-  float sum = r[0];
-#pragma unroll
-  for (int a = 1; a < K; ++a) sum += r[a];
-#pragma unroll
-  for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = r[a] / sum;
 }
 void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,
                              double inv_cc, int fullD, hipStream_t s) {
-  const int nb = nblocks(G.ncells_hash);
+  const int nb = (G.ncells_hash + kSweepSpan - 1) / kSweepSpan;
   switch (K) {
     case 1: hipLaunchKernelGGL(k_jacobi_diffusion<1>, dim3(nb), dim3(kBlock), 0, s, G, gr, pin, fullJ, fin, lambda, inv_cc, fullD); break;
     case 2: hipLaunchKernelGGL(k_jacobi_diffusion<2>, dim3(nb), dim3(kBlock), 0, s, G, gr, pin, fullJ, fin, lambda, inv_cc, fullD); break;
