@@ -44,8 +44,9 @@ STAGES = {
     "extrapolate": 8,
     "diffusion": 9,
     "pressure_diffusion": 10,  # paired pressure + diffusion sweeps (timing only)
+    "p2g": 11,  # the P2G kernel of transfer_to_grid inside a step (timing only)
 }
-STAGE_COUNT = 11
+STAGE_COUNT = 12
 
 
 class Volume(C.Structure):

@@ -85,6 +85,7 @@ struct Grid {
   float* nvf[2][4];     // newVolumeFractions, ping-pong
   uint32_t* active;     // per 256-cell launch block of the interior: any FLUID cell
                         // (32-bit so a block reads its flag with one scalar load)
+  uint32_t* alist;      // [0] = count, then the ids of the sweep chunks holding FLUID
 };
 
 struct Bins {
@@ -142,6 +143,10 @@ void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int 
                              int doBoundary, hipStream_t s);
 void launch_divergence(const Geom& G, Grid gr, int nv, Bins B, hipStream_t s);
 void launch_refresh_rhs(const Geom& G, Grid gr, hipStream_t s);
+// Rebuild gr.alist from gr.active (both launchers above end with it).
+void launch_active_list(const Geom& G, Grid gr, hipStream_t s);
+// Number of sweep chunks (the unit of gr.alist).
+int sweep_chunks(const Geom& G);
 void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 // full = 1: every cell written (AIR cells get 0); full = 0: blocks without FLUID
 // cells and AIR cells are skipped -- valid once both buffers hold 0 on AIR.

@@ -305,6 +305,7 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   // One flag per 256-cell block plus one slot that stays 0, so the sweep kernels'
   // pairs of blocks never read past the array.
   AQ_TRY(h->allocT(&gr.active, (size_t)(G.ncells_hash + 255) / 256 + 1));
+  AQ_TRY(h->allocT(&gr.alist, (size_t)sweep_chunks(G) + 1));
   for (int a = 0; a < 4; ++a) {
     AQ_TRY(h->allocT(&gr.vf[a], nc));
     AQ_TRY(h->allocT(&gr.nvf[0][a], nc));
@@ -353,11 +354,17 @@ aq_status upload_particles(aq_flip* h, const float* pos, const float* vel, const
 void content_changed(aq_flip* h) { h->air_clean[0] = h->air_clean[1] = false; }
 
 // hash is already binned (count filled): scan, scatter, segment sort, gather, P2G.
-void transfer_after_count(aq_flip* h) {
+// Inside a step the P2G kernel is timed in its own bucket (split = true).
+void transfer_after_count(aq_flip* h, bool split = false) {
   launch_scan(h->G, h->B, h->stream);
   launch_scatter_sort_gather(h->G, h->P[h->cur], h->P[h->cur ^ 1], h->Np, h->K, h->B, h->stream);
   h->cur ^= 1;
+  if (split) {
+    h->end(AQ_STAGE_TRANSFER_TO_GRID);
+    h->begin(AQ_STAGE_P2G);
+  }
   launch_p2g(h->G, h->P[h->cur], h->Np, h->K, h->B, h->gr, h->nv, h->hv, h->nf, h->sp, h->stream);
+  if (split) h->end(AQ_STAGE_P2G);
   content_changed(h);
 }
 
@@ -446,8 +453,8 @@ int stage_diffusion(aq_flip* h, bool inStep) {
 }
 
 aq_status run_stage(aq_flip* h, int st) {
-  if (st == AQ_STAGE_PRESSURE_DIFFUSION)
-    return fail(AQ_EINVAL, "AQ_STAGE_PRESSURE_DIFFUSION runs only inside a step");
+  if (st == AQ_STAGE_PRESSURE_DIFFUSION || st == AQ_STAGE_P2G)
+    return fail(AQ_EINVAL, "timing bucket: runs only inside a step");
   if (st < 0 || st >= AQ_STAGE_COUNT) return fail(AQ_EINVAL, "unknown stage");
   int64_t passes = 1;
   h->begin(st);
@@ -672,8 +679,7 @@ aq_status enqueue_step(aq_flip* h) {
   launch_g2p_advect_hash(h->G, h->P[h->cur], h->Np, h->K, h->gr, h->nv, h->nf, h->sp, h->B, h->stream);
   h->end(AQ_STAGE_G2P);
   h->begin(AQ_STAGE_TRANSFER_TO_GRID);
-  transfer_after_count(h);
-  h->end(AQ_STAGE_TRANSFER_TO_GRID);
+  transfer_after_count(h, true);  // ends the bucket before P2G, which is timed as AQ_STAGE_P2G
   h->begin(AQ_STAGE_GRAVITY);
   launch_gravity_boundary(h->G, h->gr, h->nv, h->sp, 1, 1, h->stream);
   h->end(AQ_STAGE_GRAVITY);
@@ -1551,6 +1557,11 @@ aq_status aq_flip_load(const char* path, aq_flip** out) {
   // Pressure buffers: only the current one was saved, so neither is known to hold 0
   // on every AIR cell; the next sweeps write them fully (same values either way).
   h->air_clean[0] = h->air_clean[1] = false;
+  launch_active_list(h->G, h->gr, h->stream);  // derived from the restored flags
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    delete h;
+    return bail(AQ_EHIP, "snapshot: rebuilding the active list failed");
+  }
   *out = h;
   return AQ_OK;
 }

@@ -37,36 +37,67 @@ METRIC = "FLIP steps/sec at 128^3, 8 ppc; Jacobi iters to 1e-4 residual, plain v
 NN_REFRESH = 200  # sweeps between U-Net corrections of the iterate in the warm-start solves
 
 
-def algorithmic_bytes(cfg, n_particles: int, K: int) -> dict:
-    """Bytes each kernel group must move once per launch (every operand read or
-    written exactly once), per the data layout of the library."""
+SWEEP_SPAN = 512  # cells per sweep chunk (the unit the listed sweeps skip or visit)
+
+
+def swept_cells(content, size) -> int:
+    """Cells a listed sweep visits: every cell of each 512-cell chunk (in the
+    z-fastest interior order) that holds a FLUID cell."""
+    import numpy as np
+    X, Y, Z = size
+    fluid = (content[:X, :Y, :Z] == 1).reshape(-1)
+    pad = (-fluid.size) % SWEEP_SPAN
+    chunks = np.concatenate([fluid, np.zeros(pad, bool)]).reshape(-1, SWEEP_SPAN).any(1)
+    return int(min(chunks.sum() * SWEEP_SPAN, fluid.size))
+
+
+def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps: dict) -> dict:
+    """Bytes each stage must move once per launch (every operand read or written
+    exactly once), per the data layout of the library. `sweeps` holds the launches
+    per step of the sweep stages. Sweeps count the cells a launch visits: all C
+    cells for the first two pressure sweeps of a step (both pressure buffers are
+    rewritten in full after the FLUID set changes) and for the first diffusion sweep
+    (it copies AIR cells across), the chunks holding FLUID otherwise."""
     X, Y, Z = cfg.size
     C = X * Y * Z
     Np = n_particles
+    rec = 24 + 4 * K  # particle record bytes that matter (position, velocity, K fractions)
+    paired = sweeps.get("pressure_diffusion", 0)
+    alone = sweeps.get("pressure", 0)
+    diff = sweeps.get("diffusion", 0)
+
+    def per_launch(n, full):
+        return (full * C + (n - full) * active_cells) / n if n else 0.0
+
     return {
-        # G2P + RK3 + hash + bin count: particle pos, vel r/w, fractions written, hash
-        # written, one bin increment; grid old/new face velocities and fractions read once.
-        "g2p": Np * (12 + 12 + 12 + 12 + 4 * K + 4 + 4) + C * (24 + 4 * K),
-        # scan, scatter, segment sort, gather and P2G (offsets, payload, grid outputs)
-        "transfer_to_grid": (Np * (4 + 4 + 4 + 4 + 2 * (24 + 4 * K) + 24 + 4 * K)
-                             + C * (4 * 3 + 4 + 24 + 1 + 1 + 8 * K + 4)),
+        # G2P + RK3 + hash + bin count: position and velocity read, position, velocity
+        # and fractions written, hash and rank written; grid old/new face velocities
+        # and fractions read once.
+        "g2p": Np * (24 + 24 + 4 * K + 4 + 4) + C * (24 + 4 * K),
+        # scan (counts in, offsets out), scatter (hash, rank in; permutation out),
+        # segment sort (permutation checked), gather (permutation, record in and out)
+        "transfer_to_grid": C * (4 + 4 + 4) + Np * (4 + 4 + 4) + Np * 4 + Np * (4 + 2 * rec),
+        # P2G: sorted records and cell offsets in; velocity, new velocity, valid bits,
+        # content, density and both fraction copies out
+        "p2g": Np * rec + C * (4 + 12 + 12 + 1 + 1 + 4 + 8 * K),
         "gravity": C * (1 + 12 + 12 + 1),
         "divergence": C * (12 + 4 + 4 + 4 + 4),
-        "pressure": C * (4 + 4 + 1 + 4),  # per Jacobi sweep
+        # per sweep: p 4 r + rhs 4 r + mask 1 r + p 4 w per visited cell
+        "pressure": 13 * per_launch(alone, min(alone, max(0, 2 - paired))),
         "project": C * (1 + 4 + 4 + 12 + 12 + 1),
         "extrapolate": C * (12 + 1 + 12 + 1),  # per pass
-        "diffusion": C * (1 + 8 * K),  # per sweep
+        "diffusion": (1 + 8 * K) * per_launch(diff, 1 if diff and not paired else 0),
         # one pressure sweep and one diffusion sweep in the same launch
-        "pressure_diffusion": C * (4 + 4 + 1 + 4) + C * (1 + 8 * K),
+        "pressure_diffusion": (13 + 1 + 8 * K) * per_launch(paired, min(2, paired)),
     }
 
 
 # Kernels of each stage, as rocprofv3 names them (template arguments stripped).
 STAGE_KERNELS = {
     "g2p": ["aqf::k_g2p_advect_hash"],
-    "transfer_to_grid": ["aqf::k_fill_u32", "aqf::k_scan_reduce", "aqf::k_scan_partials",
-                         "aqf::k_scan_final", "aqf::k_scatter", "aqf::k_segsort", "aqf::k_gather",
-                         "aqf::k_p2g"],
+    "transfer_to_grid": ["aqf::k_scan_reduce", "aqf::k_scan_partials", "aqf::k_scan_final",
+                         "aqf::k_scatter", "aqf::k_segsort", "aqf::k_gather"],
+    "p2g": ["aqf::k_p2g"],
     "gravity": ["aqf::k_gravity_boundary"],
     "divergence": ["aqf::k_divergence"],
     "pressure": ["aqf::k_jacobi"],
@@ -199,14 +230,18 @@ def main() -> int:
     # one kernel launch per pass for the sweep stages.
     h.reset_timing()
     h.set_timing(True)
+    active0 = swept_cells(h.get_field("content"), cfg.size)
     tp0 = time.perf_counter()
     h.steps(args.steps)
     tp1 = time.perf_counter()
     h.set_timing(False)
     timing = h.timing()
     eager_ms_per_step = 1000.0 * (tp1 - tp0) / args.steps
+    # cells in chunks with FLUID, averaged over the start and the end of the pass
+    active_cells = 0.5 * (active0 + swept_cells(h.get_field("content"), cfg.size))
 
-    nbytes = algorithmic_bytes(cfg, n, K)
+    sweeps = {name: launches / args.steps for name, (ms, launches) in timing.items()}
+    nbytes = algorithmic_bytes(cfg, n, K, active_cells, sweeps)
     kernels = {}
     for name, (ms, launches) in timing.items():
         if name not in nbytes or launches == 0:
@@ -215,7 +250,7 @@ def main() -> int:
         gbs = nbytes[name] / (avg_us * 1e-6) / 1e9
         kernels[name] = {"ms_per_step": round(ms / args.steps, 4),
                          "launches_per_step": round(launches / args.steps, 3),
-                         "avg_launch_us": round(avg_us, 3), "bytes_per_launch": nbytes[name],
+                         "avg_launch_us": round(avg_us, 3), "bytes_per_launch": round(nbytes[name]),
                          "achieved_GBs": round(gbs, 1), "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dominant]
@@ -282,6 +317,7 @@ def main() -> int:
                    "diffusion_iters": cfg.diffusion_iters, "phases": K},
         "roofline": roofline,
         "kernels": kernels,
+        "sweep_cells": {"all": cfg.size[0] * cfg.size[1] * cfg.size[2], "visited_when_listed": round(active_cells)},
         "jacobi_to_tol": solve,
         "tolerance_mode": tol_mode,
         "profiled_pass_ms_per_step": round(eager_ms_per_step, 4),

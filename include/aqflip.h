@@ -112,7 +112,10 @@ enum {
   /* Pressure and diffusion sweeps paired in one launch inside a fixed-sweep step (the
    * two chains are independent); a timing bucket only, not runnable on its own. */
   AQ_STAGE_PRESSURE_DIFFUSION = 10,
-  AQ_STAGE_COUNT = 11
+  /* The P2G kernel of transferToGrid, timed apart from the hashing and sorting
+   * inside a step; a timing bucket only (AQ_STAGE_TRANSFER_TO_GRID runs both). */
+  AQ_STAGE_P2G = 11,
+  AQ_STAGE_COUNT = 12
 };
 
 typedef struct aq_flip aq_flip; /* opaque handle */

@@ -276,7 +276,7 @@ def test_stage_timing_counts_and_matches_graph(monkeypatch):
             assert t["pressure"][1] == 2 * 30
             assert t["diffusion"][1] == 0
             assert t["extrapolate"][1] == 2 * 3
-            assert t["g2p"][1] == 2 and t["transfer_to_grid"][1] == 2
+            assert t["g2p"][1] == 2 and t["transfer_to_grid"][1] == 2 and t["p2g"][1] == 2
             assert all(ms >= 0 for ms, _ in t.values())
         out[mode] = {f: h.get_field(f) for f in ("pressure", "new_velocity", "new_volume_fractions")}
         out[mode]["pos"] = h.get_particles()[0]
