@@ -31,12 +31,12 @@ __device__ __forceinline__ int gtid() { return xcd_block() * kBlock + (int)threa
 static inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // ------------------------------------------------------------------ helpers
-// trilinearUnitKernel (Fluid/WeightKernels.cuh:131-136).
-__device__ __forceinline__ float hat_unit(float r) {
-  r = fabsf(r);
-  if (r > 1.f) return 0.f;
-  return 1.f - r;
-}
+// trilinearUnitKernel (Fluid/WeightKernels.cuh:131-136): |r| > 1 ? 0 : 1 - |r|.
+// Written as max(0, 1 - |r|), two instructions instead of three: for |r| <= 1 the
+// two agree, and for |r| > 1 the difference 1 - |r| is negative (exactly, since
+// |r| > 1 and 1 are both floats) so the max gives +0, the reference's 0. (They
+// differ only for a NaN r, which no finite particle position produces.)
+__device__ __forceinline__ float hat_unit(float r) { return fmaxf(0.f, 1.f - fabsf(r)); }
 // Correctly rounded float division by a constant d from inv = 1/d rounded to
 // double: the product x*inv carries a relative error below 2^-52, while the exact
 // quotient of two floats is never a float rounding midpoint and lies more than
@@ -469,7 +469,9 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
         };
         const int c0 = kept_before(clampi(lo - cs, 0, nst));
         const int c1 = kept_before(clampi(hi - cs, 0, nst));
-        for (int c = c0; c < c1; ++c) {
+        // One staged particle into this lane's sums; `own`: it lies in the lane's own
+        // cell (centre slab only), so it also feeds the cell-centre fractions.
+        auto accumulate = [&](int c, bool own) {
           const int q = p2g_slot(c);
           const float4 RA = sA[q];
           const float4 RB = sB[q];
@@ -483,8 +485,7 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
           w0 += wx;
           w1 += wy;
           w2 += wz;
-          // the centre slab keeps every particle, so c is its offset in the chunk
-          if (centre && cs + c >= ownLo && cs + c < ownHi) {
+          if (own) {
             const float wc = RA.z * kzC;  // cell centre (x+.5, y+.5, z+.5)
             f[0] += wc * RB.w;
             if (K == 2) f[1] += wc * sF1[q];
@@ -496,6 +497,16 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
             }
             ws += wc;
           }
+        };
+        if (!centre) {
+          for (int c = c0; c < c1; ++c) accumulate(c, false);
+        } else {
+          // the centre slab keeps every particle, so c is its offset in the chunk; the
+          // lane's range splits into before / inside / after its own cell, in order
+          const int o0 = clampi(ownLo - cs, c0, c1), o1 = clampi(ownHi - cs, o0, c1);
+          for (int c = c0; c < o0; ++c) accumulate(c, false);
+          for (int c = o0; c < o1; ++c) accumulate(c, true);
+          for (int c = o1; c < c1; ++c) accumulate(c, false);
         }
       }
     }
@@ -711,31 +722,22 @@ __device__ __forceinline__ bool chunk_active(const uint32_t* __restrict__ active
 }
 
 // The active chunks in ascending order: list[0] = count, list[1..count] = chunk
-// ids. One 1024-thread workgroup: per-thread counts over a contiguous range, an LDS
-// scan, then the writes. Rebuilt after every update of the flags.
-__global__ __launch_bounds__(1024) void k_active_list(int nchunks, const uint32_t* __restrict__ active,
-                                                      uint32_t* __restrict__ list) {
-  __shared__ uint32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (nchunks + 1023) / 1024;
-  const int c0 = min(t * per, nchunks), c1 = min(c0 + per, nchunks);
-  uint32_t n = 0;
-  for (int c = c0; c < c1; ++c) n += chunk_active(active, c) ? 1u : 0u;
-  part[t] = n;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {  // inclusive scan
-    const uint32_t v = t >= o ? part[t - o] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  uint32_t pos = part[t] - n;
+// ids. One workgroup: per-thread counts over a contiguous range of chunks, a
+// block scan, then the writes. Rebuilt after every update of the flags.
+__global__ void k_active_list(int nchunks, const uint32_t* __restrict__ active,
+                              uint32_t* __restrict__ list) {
+  const int per = (nchunks + kBlock - 1) / kBlock;
+  const int c0 = min((int)threadIdx.x * per, nchunks), c1 = min(c0 + per, nchunks);
+  int n = 0;
+  for (int c = c0; c < c1; ++c) n += chunk_active(active, c) ? 1 : 0;
+  int total;
+  int pos = block_exclusive_scan(n, &total);
   for (int c = c0; c < c1; ++c)
     if (chunk_active(active, c)) list[1 + pos++] = (uint32_t)c;
-  if (t == 1023) list[0] = part[1023];
+  if (threadIdx.x == 0) list[0] = (uint32_t)total;
 }
 void launch_active_list(const Geom& G, Grid gr, hipStream_t s) {
-  hipLaunchKernelGGL(k_active_list, dim3(1), dim3(1024), 0, s, sweep_chunks(G), gr.active, gr.alist);
+  hipLaunchKernelGGL(k_active_list, dim3(1), dim3(kBlock), 0, s, sweep_chunks(G), gr.active, gr.alist);
 }
 
 // Work split of a sweep. A full sweep (every cell written) runs one workgroup per
@@ -913,42 +915,48 @@ __global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
     return;
   }
   const float eps = (float)(-1 + 1e-6);
-  const uint8_t H = Hi[gi];
-  float val[3] = {Ui[gi], Vi[gi], Wi[gi]};
-  if (H == 7) {
-    Uo[gi] = val[0];
-    Vo[gi] = val[1];
-    Wo[gi] = val[2];
-    Ho[gi] = H;
-    return;
-  }
   const float* __restrict__ F[3] = {Ui, Vi, Wi};
   // neighbours -x, -y, -z, +x, +y, +z
   const bool ok[6] = {x > 0, y > 0, z > 0, x < G.X - 1, y < G.Y - 1, z < G.Z - 1};
   const long long off[6] = {-G.sx, -(long long)G.EZ, -1, G.sx, (long long)G.EZ, 1};
+  // Every load (the face, and the valid bits and three components of each in-range
+  // neighbour; out-of-range ones read the face itself and are ignored) is issued
+  // before any is used: one memory round trip per face.
+  const uint8_t H = Hi[gi];
+  float val[3] = {Ui[gi], Vi[gi], Wi[gi]};
+  uint32_t nh[6];
+  float nv[6][3];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const long long go = ok[k] ? gi + off[k] : gi;
+    nh[k] = Hi[go];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nv[k][c] = F[c][go];
+  }
+  asm volatile("" ::"v"(val[0]), "v"(val[1]), "v"(val[2]));
+#pragma unroll
+  for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(nh[k]), "v"(nv[k][0]), "v"(nv[k][1]), "v"(nv[k][2]));
   uint8_t Hn = H;
+  if (H != 7) {
 #pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    if (H & (1u << comp)) continue;
-    float sum = 0.f;
-    int cnt = 0;
+    for (int comp = 0; comp < 3; ++comp) {
+      if (H & (1u << comp)) continue;
+      float sum = 0.f;
+      int cnt = 0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      if (!ok[k]) continue;
-      const long long go = gi + off[k];
-      const int d = k % 3;
-      const uint8_t nh = Hi[go];
-      const float dirv = F[d][go];
-      const bool dirOk = (k < 3) ? (dirv > eps) : (dirv < -eps);
-      if ((nh & (1u << comp)) && dirOk) {
-        sum += F[comp][go];
-        cnt++;
+      for (int k = 0; k < 6; ++k) {
+        const float dirv = nv[k][k % 3];
+        const bool dirOk = (k < 3) ? (dirv > eps) : (dirv < -eps);
+        if (ok[k] && (nh[k] & (1u << comp)) && dirOk) {
+          sum += nv[k][comp];
+          cnt++;
+        }
       }
-    }
-    if (cnt > 0) {
-      const double inv_cnt[7] = {0.0, 1.0, 1.0 / 2.0, 1.0 / 3.0, 1.0 / 4.0, 1.0 / 5.0, 1.0 / 6.0};
-      val[comp] = div_exact(sum, inv_cnt[cnt]);
-      Hn |= (uint8_t)(1u << comp);
+      if (cnt > 0) {
+        const double inv_cnt[7] = {0.0, 1.0, 1.0 / 2.0, 1.0 / 3.0, 1.0 / 4.0, 1.0 / 5.0, 1.0 / 6.0};
+        val[comp] = div_exact(sum, inv_cnt[cnt]);
+        Hn |= (uint8_t)(1u << comp);
+      }
     }
   }
   Uo[gi] = val[0];
