@@ -150,7 +150,8 @@ int sweep_chunks(const Geom& G);
 void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 // full = 1: every cell written (AIR cells get 0); full = 0: blocks without FLUID
 // cells and AIR cells are skipped -- valid once both buffers hold 0 on AIR.
-void launch_jacobi(const Geom& G, Grid gr, int pin, int full, hipStream_t s);
+// pzero = 1: the input is the zero guess (not read); needs full = 1.
+void launch_jacobi(const Geom& G, Grid gr, int pin, int full, int pzero, hipStream_t s);
 void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s);
 void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s);
@@ -161,7 +162,7 @@ void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, doub
 // One Jacobi sweep (p[pin] -> p[pin^1]) and one diffusion sweep (nvf[fin] ->
 // nvf[fin^1]) in one launch; fullJ / fullD as `full` above.
 void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,
-                             double inv_cc, int fullD, hipStream_t s);
+                             double inv_cc, int fullD, int pzero, hipStream_t s);
 void launch_g2p(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv, int nf,
                 StepParams sp, hipStream_t s);
 void launch_advect(const Geom& G, Particles P, long long Np, Grid gr, int nv, StepParams sp,

@@ -379,14 +379,19 @@ void stage_transfer(aq_flip* h) {
 // n Jacobi sweeps. Every sweep leaves 0 on all AIR cells of its output buffer, so
 // a sweep whose output buffer already holds those zeros may skip AIR cells and
 // blocks without FLUID (the result is identical); otherwise it writes every cell.
-void jacobi_sweeps(aq_flip* h, int n) {
+// from_zero: the sweeps start from the zero guess; the first one does not read the
+// input buffer (its neighbours are 0) and writes every cell, so neither buffer needs
+// clearing first.
+void jacobi_sweeps(aq_flip* h, int n, bool from_zero = false) {
   for (int i = 0; i < n; ++i) {
     const int out = h->pb ^ 1;
-    launch_jacobi(h->G, h->gr, h->pb, h->air_clean[out] ? 0 : 1, h->stream);
+    const bool z = from_zero && i == 0;
+    launch_jacobi(h->G, h->gr, h->pb, (z || !h->air_clean[out]) ? 1 : 0, z ? 1 : 0, h->stream);
     h->air_clean[out] = true;
     h->pb = out;
   }
 }
+
 
 // Zero pressure start: both buffers cleared.
 void zero_pressure(aq_flip* h) {
@@ -394,11 +399,14 @@ void zero_pressure(aq_flip* h) {
   launch_fill_pressure_zero(h->G, h->gr, h->pb ^ 1, h->stream);
   h->air_clean[0] = h->air_clean[1] = true;
 }
-
-void stage_pressure(aq_flip* h) {
-  if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) zero_pressure(h);
-  jacobi_sweeps(h, h->cfg.pressure_iters);
+// The fixed-sweep solve: pressure_iters sweeps from the configured start.
+void fixed_sweeps(aq_flip* h) {
+  const bool zero = h->cfg.pressure_init == AQ_PRESSURE_ZERO;
+  if (zero && h->cfg.pressure_iters == 0) zero_pressure(h);
+  jacobi_sweeps(h, h->cfg.pressure_iters, zero);
 }
+
+void stage_pressure(aq_flip* h) { fixed_sweeps(h); }
 
 void stage_extrapolate(aq_flip* h) {
   for (int i = 0; i < 3; ++i) {  // maxDistance = 3 (Fluid_3D_common.cu:74)
@@ -420,13 +428,14 @@ void diffusion_sweeps(aq_flip* h, hipStream_t s, int first = 0) {
 // min(pressure_iters, diffusion_iters) sweeps of each run pairwise in one launch,
 // the rest of the longer chain on its own. Same iterates as the two loops.
 void fused_pressure_diffusion(aq_flip* h) {
-  if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) zero_pressure(h);
+  const bool zero = h->cfg.pressure_init == AQ_PRESSURE_ZERO;  // first sweep from the zero guess
   const int n = std::min(h->cfg.pressure_iters, h->cfg.diffusion_iters);
   h->begin(AQ_STAGE_PRESSURE_DIFFUSION);
   for (int i = 0; i < n; ++i) {
     const int out = h->pb ^ 1;
-    launch_jacobi_diffusion(h->G, h->gr, h->K, h->pb, h->air_clean[out] ? 0 : 1, h->nf, h->sp.lambda,
-                            h->sp.inv_cc, i == 0 ? 1 : 0, h->stream);
+    const bool z = zero && i == 0;
+    launch_jacobi_diffusion(h->G, h->gr, h->K, h->pb, (z || !h->air_clean[out]) ? 1 : 0, h->nf,
+                            h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, z ? 1 : 0, h->stream);
     h->air_clean[out] = true;
     h->pb = out;
     h->nf ^= 1;
@@ -607,9 +616,8 @@ aq_status sweep_to_tol(aq_flip* h, int budget, bool detect_stall, int* sweeps, d
 aq_status pressure_stage(aq_flip* h) {
   aq_flip_solver_stats& st = h->sstats;
   if (h->solver.mode == AQ_SOLVE_FIXED) {
-    if (h->cfg.pressure_init == AQ_PRESSURE_ZERO) zero_pressure(h);
-    h->begin(AQ_STAGE_PRESSURE);  // the Jacobi sweeps only
-    jacobi_sweeps(h, h->cfg.pressure_iters);
+    h->begin(AQ_STAGE_PRESSURE);  // the Jacobi sweeps
+    fixed_sweeps(h);
     h->end(AQ_STAGE_PRESSURE, h->cfg.pressure_iters);
     return AQ_OK;
   }

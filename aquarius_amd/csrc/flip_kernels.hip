@@ -707,8 +707,12 @@ void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
 // 4.83 us and a paired sweep from 8.75 to 8.31 us; four were slower (6.1 us).
 constexpr int kSweepCells = 2;
 constexpr int kSweepSpan = kSweepCells * kBlock;  // cells per chunk (one workgroup's work)
-// Workgroups of a listed sweep (a multiple of kXCD): they stride over the active list.
+// Workgroups of a listed sweep (multiples of kXCD): they stride over the active list.
+// Measured at 128^3 (1,441 of 4,096 chunks active): a pressure sweep is fastest with
+// 1024 (3.59 us; 2048: 3.79), a paired sweep with 1536-2048 (6.19 us; 1024: 6.24,
+// 512: 7.28).
 constexpr int kListedGrid = 1024;
+constexpr int kListedGridPaired = 2048;
 
 int sweep_chunks(const Geom& G) { return (G.ncells_hash + kSweepSpan - 1) / kSweepSpan; }
 
@@ -746,10 +750,10 @@ void launch_active_list(const Geom& G, Grid gr, hipStream_t s) {
 // the list, its workgroups striding through it, so a sweep costs workgroups only
 // where FLUID is and each XCD's L2 sees one contiguous range of the grid. At 128^3
 // (35 % of the chunks active) this took a pressure sweep from 4.88 to 3.62 us.
-int sweep_grid(const Geom& G, bool full) {
+int sweep_grid(const Geom& G, bool full, int listed = kListedGrid) {
   const int nch = sweep_chunks(G);
   if (full || nch < kXCD) return nch;
-  return std::min(nch, kListedGrid) / kXCD * kXCD;
+  return std::min(nch, listed) / kXCD * kXCD;
 }
 template <typename F>
 __device__ __forceinline__ void for_each_chunk(const uint32_t* __restrict__ list, bool full, F&& body) {
@@ -803,7 +807,9 @@ __device__ __forceinline__ float jacobi_value(float b, const float v[6]) {
 // before any is used (an empty `asm volatile` consumes them, so the compiler cannot
 // sink loads behind a branch), so a cell costs one memory round trip. AIR cells are
 // always written (0): what a full sweep needs and what a listed sweep's buffer
-// already holds; a listed sweep visits only the chunks with FLUID.
+// already holds; a listed sweep visits only the chunks with FLUID. With `pzero` the
+// input is the zero guess: the neighbours are 0 and are not read (same sums).
+template <bool PZERO>
 __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
                          float* __restrict__ pout, const uint32_t* __restrict__ list, int full) {
   for_each_chunk(list, full != 0, [&](int ch) {
@@ -814,7 +820,7 @@ __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __r
       q[c] = sweep_cell(G, ch, c);
       b[c] = rhs[q[c].gi];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) v[c][k] = pin[q[c].nb[k]];
+      for (int k = 0; k < 6; ++k) v[c][k] = PZERO ? 0.f : pin[q[c].nb[k]];
     }
 #pragma unroll
     for (int c = 0; c < kSweepCells; ++c)
@@ -825,9 +831,14 @@ __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __r
       if (q[c].ok) pout[q[c].gi] = jacobi_value(b[c], v[c]);
   });
 }
-void launch_jacobi(const Geom& G, Grid gr, int pin, int full, hipStream_t s) {
-  hipLaunchKernelGGL(k_jacobi, dim3(sweep_grid(G, full != 0)), dim3(kBlock), 0, s, G, gr.rhs,
-                     gr.p[pin], gr.p[pin ^ 1], gr.alist, full);
+void launch_jacobi(const Geom& G, Grid gr, int pin, int full, int pzero, hipStream_t s) {
+  const dim3 grid(sweep_grid(G, full != 0));
+  if (pzero)
+    hipLaunchKernelGGL(k_jacobi<true>, grid, dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin], gr.p[pin ^ 1],
+                       gr.alist, 1);
+  else
+    hipLaunchKernelGGL(k_jacobi<false>, grid, dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin], gr.p[pin ^ 1],
+                       gr.alist, full);
 }
 
 // diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
@@ -1047,7 +1058,7 @@ void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, doub
 // shared cell and neighbour indices once. Every load of both is issued before any
 // is used, so the cells still cost one memory round trip. fullJ and fullD mean what
 // `full` means for each sweep.
-template <int K>
+template <int K, bool PZERO>
 __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int pin, int fin, float lambda,
                                              double inv_cc, bool fullD, bool doJ, bool doD, int ch) {
   const float* __restrict__ P = gr.p[pin];
@@ -1059,7 +1070,7 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int 
     q[c] = sweep_cell(G, ch, c);
     b[c] = gr.rhs[q[c].gi];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) pv[c][k] = P[q[c].nb[k]];
+    for (int k = 0; k < 6; ++k) pv[c][k] = PZERO ? 0.f : P[q[c].nb[k]];
     cc[c] = gr.content[q[c].gi];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
@@ -1109,7 +1120,7 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int 
     for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = r[a] / sum;
   }
 }
-template <int K>
+template <int K, bool PZERO>
 __global__ void k_jacobi_diffusion(Geom G, Grid gr, int pin, int fullJ, int fin, float lambda,
                                    double inv_cc, int fullD) {
   // Either sweep full: every chunk, each part where its own rule asks; otherwise the
@@ -1118,17 +1129,27 @@ __global__ void k_jacobi_diffusion(Geom G, Grid gr, int pin, int fullJ, int fin,
   for_each_chunk(gr.alist, full, [&](int ch) {
     const bool act = !full || chunk_active(gr.active, ch);
     const bool doJ = fullJ || act, doD = fullD || act;
-    if (doJ || doD) paired_chunk<K>(G, gr, pin, fin, lambda, inv_cc, fullD != 0, doJ, doD, ch);
+    if (doJ || doD) paired_chunk<K, PZERO>(G, gr, pin, fin, lambda, inv_cc, fullD != 0, doJ, doD, ch);
   });
 }
+template <int K>
+void launch_paired(const Geom& G, Grid gr, int pin, int fullJ, int fin, float lambda, double inv_cc,
+                   int fullD, int pzero, hipStream_t s) {
+  const dim3 grid(sweep_grid(G, fullJ || fullD, kListedGridPaired));
+  if (pzero)
+    hipLaunchKernelGGL((k_jacobi_diffusion<K, true>), grid, dim3(kBlock), 0, s, G, gr, pin, fullJ, fin,
+                       lambda, inv_cc, fullD);
+  else
+    hipLaunchKernelGGL((k_jacobi_diffusion<K, false>), grid, dim3(kBlock), 0, s, G, gr, pin, fullJ, fin,
+                       lambda, inv_cc, fullD);
+}
 void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,
-                             double inv_cc, int fullD, hipStream_t s) {
-  const int nb = sweep_grid(G, fullJ || fullD);
+                             double inv_cc, int fullD, int pzero, hipStream_t s) {
   switch (K) {
-    case 1: hipLaunchKernelGGL(k_jacobi_diffusion<1>, dim3(nb), dim3(kBlock), 0, s, G, gr, pin, fullJ, fin, lambda, inv_cc, fullD); break;
-    case 2: hipLaunchKernelGGL(k_jacobi_diffusion<2>, dim3(nb), dim3(kBlock), 0, s, G, gr, pin, fullJ, fin, lambda, inv_cc, fullD); break;
-    case 3: hipLaunchKernelGGL(k_jacobi_diffusion<3>, dim3(nb), dim3(kBlock), 0, s, G, gr, pin, fullJ, fin, lambda, inv_cc, fullD); break;
-    default: hipLaunchKernelGGL(k_jacobi_diffusion<4>, dim3(nb), dim3(kBlock), 0, s, G, gr, pin, fullJ, fin, lambda, inv_cc, fullD); break;
+    case 1: launch_paired<1>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, s); break;
+    case 2: launch_paired<2>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, s); break;
+    case 3: launch_paired<3>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, s); break;
+    default: launch_paired<4>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, s); break;
   }
 }
 
