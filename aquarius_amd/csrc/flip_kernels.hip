@@ -91,23 +91,31 @@ __device__ __forceinline__ int cell_hash(const Geom& G, float px, float py, floa
 // Bin one particle per lane: lanes of the wave holding the same hash reserve their
 // slots with one atomic (issued by the lowest such lane) and take consecutive
 // ranks in lane order. rank = slot of the particle inside its cell's segment
-// before the segment sort.
+// before the segment sort. The groups are found first (ballots and shuffles only),
+// then every group leader issues its atomic in the same instruction, so a wave
+// waits for one atomic round trip however many cells its particles span.
 __device__ __forceinline__ void bin_particle(int hh, bool valid, int* __restrict__ count,
                                              int& rank) {
   const int lane = (int)(threadIdx.x & 63);
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long pending = __ballot(valid);
-  rank = 0;
+  int leader = lane;
+  unsigned long long group = 0;
   while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const int lh = __shfl(hh, leader, 64);
-    const unsigned long long same = __ballot(valid && hh == lh) & pending;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&count[lh], __popcll(same));
-    base = __shfl(base, leader, 64);
-    if (valid && hh == lh) rank = base + __popcll(same & lt);
+    const int l = __ffsll((long long)pending) - 1;
+    const int lh = __shfl(hh, l, 64);
+    const bool mine = valid && hh == lh;
+    const unsigned long long same = __ballot(mine) & pending;
+    if (mine) {
+      leader = l;
+      group = same;
+    }
     pending &= ~same;
   }
+  int base = 0;
+  if (valid && lane == leader) base = atomicAdd(&count[hh], __popcll(group));
+  base = __shfl(base, leader, 64);
+  rank = base + __popcll(group & lt);
 }
 
 __global__ void k_hash_bin(Geom G, Particles P, long long Np, int* __restrict__ hash,
@@ -429,6 +437,10 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
         int nkept = 0;
 #pragma unroll
         for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
+          if (bi * kP2GWave >= nst) {  // past the chunk's particles (wave-uniform)
+            kept[bi] = 0;
+            continue;
+          }
           const int k = bi * kP2GWave + lane;
           bool keep = false;
           float4 A = {}, B = {}, Cc = {};
