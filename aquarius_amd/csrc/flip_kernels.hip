@@ -7,6 +7,8 @@
 // reference's racy in-place texture/surface update.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "aqflip_internal.h"
 
 #pragma clang fp contract(off)
@@ -247,12 +249,24 @@ __global__ void k_scatter(const int* __restrict__ hash, const int* __restrict__ 
   perm[offsets[hash[i]] + rank[i]] = i;
 }
 
+// Segments of up to kSegFast entries are checked from registers loaded in one
+// round trip (predicated loads); longer ones by a loop. An unsorted segment is
+// insertion-sorted in place.
+constexpr int kSegFast = 16;
 __global__ void k_segsort(const int* __restrict__ offsets, int ncells, int* __restrict__ perm) {
   const int c = gtid();
   if (c >= ncells) return;
   const int s = offsets[c], e = offsets[c + 1];
   bool sorted = true;
-  for (int i = s + 1; i < e; ++i) sorted &= perm[i - 1] < perm[i];
+  if (e - s <= kSegFast) {
+    int v[kSegFast];
+#pragma unroll
+    for (int k = 0; k < kSegFast; ++k) v[k] = s + k < e ? perm[s + k] : INT_MAX;
+#pragma unroll
+    for (int k = 1; k < kSegFast; ++k) sorted &= s + k >= e || v[k - 1] < v[k];
+  } else {
+    for (int i = s + 1; i < e; ++i) sorted &= perm[i - 1] < perm[i];
+  }
   if (sorted) return;
   for (int i = s + 1; i < e; ++i) {
     const int v = perm[i];
@@ -738,22 +752,53 @@ __device__ __forceinline__ bool chunk_active(const uint32_t* __restrict__ active
 }
 
 // The active chunks in ascending order: list[0] = count, list[1..count] = chunk
-// ids. One workgroup: per-thread counts over a contiguous range of chunks, a
-// block scan, then the writes. Rebuilt after every update of the flags.
-__global__ void k_active_list(int nchunks, const uint32_t* __restrict__ active,
-                              uint32_t* __restrict__ list) {
-  const int per = (nchunks + kBlock - 1) / kBlock;
-  const int c0 = min((int)threadIdx.x * per, nchunks), c1 = min(c0 + per, nchunks);
+// ids. One 1024-thread workgroup: each thread takes a contiguous range of chunks and
+// reads their flags into registers in batches of kListBatch (one round trip per
+// batch), then a wave-shuffle scan of the per-thread counts places its ids.
+// Rebuilt after every update of the flags.
+constexpr int kListThreads = 1024;
+constexpr int kListBatch = 8;
+__global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const uint32_t* __restrict__ active,
+                                                              uint32_t* __restrict__ list) {
+  __shared__ int wsum[kListThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int per = (nchunks + kListThreads - 1) / kListThreads;
+  const int c0 = min(t * per, nchunks), c1 = min(c0 + per, nchunks);
   int n = 0;
-  for (int c = c0; c < c1; ++c) n += chunk_active(active, c) ? 1 : 0;
-  int total;
-  int pos = block_exclusive_scan(n, &total);
-  for (int c = c0; c < c1; ++c)
-    if (chunk_active(active, c)) list[1 + pos++] = (uint32_t)c;
-  if (threadIdx.x == 0) list[0] = (uint32_t)total;
+  for (int b = c0; b < c1; b += kListBatch) {
+    bool on[kListBatch];
+#pragma unroll
+    for (int k = 0; k < kListBatch; ++k) on[k] = b + k < c1 && chunk_active(active, b + k);
+#pragma unroll
+    for (int k = 0; k < kListBatch; ++k) n += on[k] ? 1 : 0;
+  }
+  int x = n;  // inclusive scan within the wave, then across the waves
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  int base = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kListThreads / 64; ++w) {
+    base += w < wid ? wsum[w] : 0;
+    total += wsum[w];
+  }
+  int pos = base + x - n;
+  for (int b = c0; b < c1; b += kListBatch) {
+    bool on[kListBatch];
+#pragma unroll
+    for (int k = 0; k < kListBatch; ++k) on[k] = b + k < c1 && chunk_active(active, b + k);
+#pragma unroll
+    for (int k = 0; k < kListBatch; ++k)
+      if (on[k]) list[1 + pos++] = (uint32_t)(b + k);
+  }
+  if (t == 0) list[0] = (uint32_t)total;
 }
 void launch_active_list(const Geom& G, Grid gr, hipStream_t s) {
-  hipLaunchKernelGGL(k_active_list, dim3(1), dim3(kBlock), 0, s, sweep_chunks(G), gr.active, gr.alist);
+  hipLaunchKernelGGL(k_active_list, dim3(1), dim3(kListThreads), 0, s, sweep_chunks(G), gr.active, gr.alist);
 }
 
 // Work split of a sweep. A full sweep (every cell written) runs one workgroup per
