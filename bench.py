@@ -119,13 +119,18 @@ def measured_traffic(scene: str, stage: str):
     if start < 0:
         return None
     data = json.loads(text[start + 7:text.find("```", start + 7)])
-    total, found = 0, False
+    # Instantiations of one kernel (template arguments) are averaged, weighted by
+    # their dispatches; the different kernels of a stage add up.
+    per_kernel = {}
     for name, row in data["kernels"].items():
         base = name.split("<")[0]
         if base in STAGE_KERNELS.get(stage, []):
-            total += row["fetch_bytes"] + row["write_bytes"]
-            found = True
-    return total if found else None
+            b, n = per_kernel.get(base, (0.0, 0))
+            d = row["dispatches"]
+            per_kernel[base] = (b + d * (row["fetch_bytes"] + row["write_bytes"]), n + d)
+    if not per_kernel:
+        return None
+    return round(sum(b / n for b, n in per_kernel.values()))
 
 
 def load_warmstart_net():
