@@ -159,6 +159,7 @@ PROTOTYPES = {
     "aq_flip_set_timing": (C.c_int, [_P, C.c_int32]),
     "aq_flip_get_timing": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "aq_flip_reset_timing": (C.c_int, [_P]),
+    "aq_flip_paired_diffusion_computed": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "aq_flip_physical_time": (C.c_double, [_P]),
     "aq_flip_get_center": (C.c_int, [_P, _FP]),
     "aq_flip_container_size": (C.c_float, [_P]),

@@ -86,6 +86,8 @@ struct Grid {
   uint32_t* active;     // per 256-cell launch block of the interior: any FLUID cell
                         // (32-bit so a block reads its flag with one scalar load)
   uint32_t* alist;      // [0] = count, then the ids of the sweep chunks holding FLUID
+  uint32_t* dchange;    // words 0-2: paired diffusion sweep i changed a bit -> word i % 3 != 0;
+                        // word 3: paired diffusion sweeps computed (not skipped)
 };
 
 struct Bins {
@@ -161,8 +163,10 @@ void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, doub
                       hipStream_t s);
 // One Jacobi sweep (p[pin] -> p[pin^1]) and one diffusion sweep (nvf[fin] ->
 // nvf[fin^1]) in one launch; fullJ / fullD as `full` above.
+// sweep = index of the pair within the step (0 first); skip_ok = 1 lets a diffusion
+// sweep whose predecessor changed nothing skip its (identical) work.
 void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,
-                             double inv_cc, int fullD, int pzero, hipStream_t s);
+                             double inv_cc, int fullD, int pzero, int sweep, int skip_ok, hipStream_t s);
 void launch_g2p(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv, int nf,
                 StepParams sp, hipStream_t s);
 void launch_advect(const Geom& G, Particles P, long long Np, Grid gr, int nv, StepParams sp,

@@ -78,6 +78,8 @@ struct aq_flip {
   // With the fixed-sweep solver, diffusion sweep i runs in the same launch as
   // pressure sweep i (k_jacobi_diffusion) instead of on the second stream.
   bool fuse = true;
+  // Paired diffusion sweeps after one that changed no bit skip their identical work.
+  bool skip_fixed = true;
   std::vector<void*> allocs;
 
   Particles P[2]{};
@@ -306,6 +308,7 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   // pairs of blocks never read past the array.
   AQ_TRY(h->allocT(&gr.active, (size_t)(G.ncells_hash + 255) / 256 + 1));
   AQ_TRY(h->allocT(&gr.alist, (size_t)sweep_chunks(G) + 1));
+  AQ_TRY(h->allocT(&gr.dchange, 4));
   for (int a = 0; a < 4; ++a) {
     AQ_TRY(h->allocT(&gr.vf[a], nc));
     AQ_TRY(h->allocT(&gr.nvf[0][a], nc));
@@ -435,7 +438,8 @@ void fused_pressure_diffusion(aq_flip* h) {
     const int out = h->pb ^ 1;
     const bool z = zero && i == 0;
     launch_jacobi_diffusion(h->G, h->gr, h->K, h->pb, (z || !h->air_clean[out]) ? 1 : 0, h->nf,
-                            h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, z ? 1 : 0, h->stream);
+                            h->sp.lambda, h->sp.inv_cc, i == 0 ? 1 : 0, z ? 1 : 0, i,
+                            h->skip_fixed ? 1 : 0, h->stream);
     h->air_clean[out] = true;
     h->pb = out;
     h->nf ^= 1;
@@ -900,11 +904,13 @@ aq_status write_seeded(aq_flip* h, const SeedPlan& plan) {
 
 // Switches for measurements: AQFLIP_NO_GRAPH=1 (eager launches), AQFLIP_NO_OVERLAP=1
 // (diffusion after the pressure solve on one stream), AQFLIP_NO_FUSE=1 (no fused
-// pressure + diffusion sweeps). Results are identical in every combination.
+// pressure + diffusion sweeps), AQFLIP_NO_FIXED_POINT_SKIP=1 (paired diffusion sweeps
+// always computed). Results are identical in every combination.
 void read_env(aq_flip* h) {
   if (const char* env = std::getenv("AQFLIP_NO_GRAPH")) h->use_graphs = env[0] == '0';
   if (const char* env = std::getenv("AQFLIP_NO_OVERLAP")) h->overlap = env[0] != '1';
   if (const char* env = std::getenv("AQFLIP_NO_FUSE")) h->fuse = env[0] != '1';
+  if (const char* env = std::getenv("AQFLIP_NO_FIXED_POINT_SKIP")) h->skip_fixed = env[0] != '1';
 }
 
 aq_status create_common(const aq_flip_config* cfg, const float* pos, const float* vel,
@@ -1600,6 +1606,15 @@ aq_status aq_flip_get_timing(aq_flip* h, double* stage_ms, int64_t* stage_calls)
   }
   return AQ_OK;
 }
+aq_status aq_flip_paired_diffusion_computed(aq_flip* h, int64_t* count) {
+  if (!h || !count) return fail(AQ_EINVAL, "null argument");
+  uint32_t v = 0;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemcpy(&v, h->gr.dchange + 3, sizeof(v), hipMemcpyDeviceToHost));
+  *count = (int64_t)v;
+  return AQ_OK;
+}
+
 aq_status aq_flip_reset_timing(aq_flip* h) {
   if (!h) return fail(AQ_EINVAL, "null handle");
   for (int i = 0; i < AQ_STAGE_COUNT; ++i) {

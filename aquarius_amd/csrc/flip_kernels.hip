@@ -759,7 +759,8 @@ __device__ __forceinline__ bool chunk_active(const uint32_t* __restrict__ active
 constexpr int kListThreads = 1024;
 constexpr int kListBatch = 8;
 __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const uint32_t* __restrict__ active,
-                                                              uint32_t* __restrict__ list) {
+                                                              uint32_t* __restrict__ list,
+                                                              uint32_t* __restrict__ dchange) {
   __shared__ int wsum[kListThreads / 64];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int per = (nchunks + kListThreads - 1) / kListThreads;
@@ -796,9 +797,11 @@ __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const
       if (on[k]) list[1 + pos++] = (uint32_t)(b + k);
   }
   if (t == 0) list[0] = (uint32_t)total;
+  if (t == 0) dchange[0] = 0u;  // the first paired sweep of the next step records here
 }
 void launch_active_list(const Geom& G, Grid gr, hipStream_t s) {
-  hipLaunchKernelGGL(k_active_list, dim3(1), dim3(kListThreads), 0, s, sweep_chunks(G), gr.active, gr.alist);
+  hipLaunchKernelGGL(k_active_list, dim3(1), dim3(kListThreads), 0, s, sweep_chunks(G), gr.active, gr.alist,
+                     gr.dchange);
 }
 
 // Work split of a sweep. A full sweep (every cell written) runs one workgroup per
@@ -867,26 +870,30 @@ __device__ __forceinline__ float jacobi_value(float b, const float v[6]) {
 // already holds; a listed sweep visits only the chunks with FLUID. With `pzero` the
 // input is the zero guess: the neighbours are 0 and are not read (same sums).
 template <bool PZERO>
+__device__ __forceinline__ void jacobi_chunk(const Geom& G, const float* __restrict__ rhs,
+                                             const float* __restrict__ pin, float* __restrict__ pout,
+                                             int ch) {
+  SweepCell q[kSweepCells];
+  float b[kSweepCells], v[kSweepCells][6];
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c) {
+    q[c] = sweep_cell(G, ch, c);
+    b[c] = rhs[q[c].gi];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[c][k] = PZERO ? 0.f : pin[q[c].nb[k]];
+  }
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c)
+    asm volatile("" ::"v"(b[c]), "v"(v[c][0]), "v"(v[c][1]), "v"(v[c][2]), "v"(v[c][3]),
+                 "v"(v[c][4]), "v"(v[c][5]));
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c)
+    if (q[c].ok) pout[q[c].gi] = jacobi_value(b[c], v[c]);
+}
+template <bool PZERO>
 __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
                          float* __restrict__ pout, const uint32_t* __restrict__ list, int full) {
-  for_each_chunk(list, full != 0, [&](int ch) {
-    SweepCell q[kSweepCells];
-    float b[kSweepCells], v[kSweepCells][6];
-#pragma unroll
-    for (int c = 0; c < kSweepCells; ++c) {
-      q[c] = sweep_cell(G, ch, c);
-      b[c] = rhs[q[c].gi];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) v[c][k] = PZERO ? 0.f : pin[q[c].nb[k]];
-    }
-#pragma unroll
-    for (int c = 0; c < kSweepCells; ++c)
-      asm volatile("" ::"v"(b[c]), "v"(v[c][0]), "v"(v[c][1]), "v"(v[c][2]), "v"(v[c][3]),
-                   "v"(v[c][4]), "v"(v[c][5]));
-#pragma unroll
-    for (int c = 0; c < kSweepCells; ++c)
-      if (q[c].ok) pout[q[c].gi] = jacobi_value(b[c], v[c]);
-  });
+  for_each_chunk(list, full != 0, [&](int ch) { jacobi_chunk<PZERO>(G, rhs, pin, pout, ch); });
 }
 void launch_jacobi(const Geom& G, Grid gr, int pin, int full, int pzero, hipStream_t s) {
   const dim3 grid(sweep_grid(G, full != 0));
@@ -1117,7 +1124,8 @@ void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, doub
 // `full` means for each sweep.
 template <int K, bool PZERO>
 __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int pin, int fin, float lambda,
-                                             double inv_cc, bool fullD, bool doJ, bool doD, int ch) {
+                                             double inv_cc, bool fullD, bool doJ, bool doD, int ch,
+                                             bool& changed) {
   const float* __restrict__ P = gr.p[pin];
   SweepCell q[kSweepCells];
   float b[kSweepCells], pv[kSweepCells][6], v[kSweepCells][7][K];
@@ -1174,39 +1182,65 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int 
 #pragma unroll
     for (int a = 1; a < K; ++a) sum += r[a];
 #pragma unroll
-    for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = r[a] / sum;
+    for (int a = 0; a < K; ++a) {
+      const float f = r[a] / sum;
+      changed |= __float_as_uint(f) != __float_as_uint(v[c][6][a]);
+      gr.nvf[fin ^ 1][a][gi] = f;
+    }
   }
 }
+// Pressure sweep i paired with diffusion sweep i (`sweep` = i). A diffusion sweep that
+// changes no bit leaves both fraction buffers equal (the skipped chunks are AIR,
+// identical in both after the first sweep copied them), and every later sweep then
+// recomputes the same values from the same inputs; so each sweep records whether it
+// changed anything (gr.dchange[i % 3]) and a sweep whose predecessor changed nothing
+// skips its diffusion part: the results are those of running it. Sweep i also clears
+// the word sweep i + 1 will record into (word 0 is cleared by k_active_list, before
+// each step's first sweep). Word 3 counts the paired sweeps that computed diffusion.
 template <int K, bool PZERO>
 __global__ void k_jacobi_diffusion(Geom G, Grid gr, int pin, int fullJ, int fin, float lambda,
-                                   double inv_cc, int fullD) {
+                                   double inv_cc, int fullD, int sweep, int skip_ok) {
+  const bool skipD = skip_ok && !fullD && sweep > 0 && gr.dchange[(sweep - 1) % 3] == 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    gr.dchange[(sweep + 1) % 3] = 0u;
+    if (!skipD) atomicAdd(&gr.dchange[3], 1u);  // paired diffusion sweeps computed
+  }
   // Either sweep full: every chunk, each part where its own rule asks; otherwise the
   // active list, both parts.
   const bool full = fullJ || fullD;
+  bool changed = false;
   for_each_chunk(gr.alist, full, [&](int ch) {
     const bool act = !full || chunk_active(gr.active, ch);
     const bool doJ = fullJ || act, doD = fullD || act;
-    if (doJ || doD) paired_chunk<K, PZERO>(G, gr, pin, fin, lambda, inv_cc, fullD != 0, doJ, doD, ch);
+    if (skipD) {  // the pressure sweep alone: no diffusion loads
+      if (doJ) jacobi_chunk<PZERO>(G, gr.rhs, gr.p[pin], gr.p[pin ^ 1], ch);
+    } else if (doJ || doD) {
+      paired_chunk<K, PZERO>(G, gr, pin, fin, lambda, inv_cc, fullD != 0, doJ, doD, ch, changed);
+    }
   });
+  // One plain store of 1 per wave that changed something: every writer stores the same
+  // value, so the race is benign (a same-address atomic from every wave serialises: 39 us
+  // per sweep in a mixing scene).
+  if (__ballot(changed) && (threadIdx.x & 63) == 0) gr.dchange[sweep % 3] = 1u;
 }
 template <int K>
 void launch_paired(const Geom& G, Grid gr, int pin, int fullJ, int fin, float lambda, double inv_cc,
-                   int fullD, int pzero, hipStream_t s) {
+                   int fullD, int pzero, int sweep, int skip_ok, hipStream_t s) {
   const dim3 grid(sweep_grid(G, fullJ || fullD, kListedGridPaired));
   if (pzero)
     hipLaunchKernelGGL((k_jacobi_diffusion<K, true>), grid, dim3(kBlock), 0, s, G, gr, pin, fullJ, fin,
-                       lambda, inv_cc, fullD);
+                       lambda, inv_cc, fullD, sweep, skip_ok);
   else
     hipLaunchKernelGGL((k_jacobi_diffusion<K, false>), grid, dim3(kBlock), 0, s, G, gr, pin, fullJ, fin,
-                       lambda, inv_cc, fullD);
+                       lambda, inv_cc, fullD, sweep, skip_ok);
 }
 void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,
-                             double inv_cc, int fullD, int pzero, hipStream_t s) {
+                             double inv_cc, int fullD, int pzero, int sweep, int skip_ok, hipStream_t s) {
   switch (K) {
-    case 1: launch_paired<1>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, s); break;
-    case 2: launch_paired<2>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, s); break;
-    case 3: launch_paired<3>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, s); break;
-    default: launch_paired<4>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, s); break;
+    case 1: launch_paired<1>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, sweep, skip_ok, s); break;
+    case 2: launch_paired<2>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, sweep, skip_ok, s); break;
+    case 3: launch_paired<3>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, sweep, skip_ok, s); break;
+    default: launch_paired<4>(G, gr, pin, fullJ, fin, lambda, inv_cc, fullD, pzero, sweep, skip_ok, s); break;
   }
 }
 

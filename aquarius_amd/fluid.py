@@ -195,6 +195,12 @@ class FlipHandle:
         check(self.lib.aq_flip_get_timing(self.h, ms, calls))
         return {name: (ms[i], calls[i]) for name, i in STAGES.items()}
 
+    def paired_diffusion_computed(self) -> int:
+        """Paired diffusion sweeps computed (not skipped as unchanged) since creation."""
+        n = C.c_int64()
+        check(self.lib.aq_flip_paired_diffusion_computed(self.h, C.byref(n)))
+        return n.value
+
     @property
     def physical_time(self) -> float:
         return float(self.lib.aq_flip_physical_time(self.h))

@@ -18,6 +18,10 @@ Prints one JSON line with the contract fields plus:
                 the sweeps from the U-Net guess to the same residual
   tolerance_mode steps/s when every step solves to 1e-4: plain Jacobi from zero
                 against the U-Net warm start
+  paired_diffusion  diffusion sweeps paired with pressure sweeps per step, how many of
+                them computed their diffusion part in the profiled pass (a sweep after
+                one that changed no bit skips the identical work), and the steps/s of
+                the timed region with that skipping switched off
   cpu_baseline  the CPU oracle timed on a bounded sample of the same workload
 """
 from __future__ import annotations
@@ -51,13 +55,16 @@ def swept_cells(content, size) -> int:
     return int(min(chunks.sum() * SWEEP_SPAN, fluid.size))
 
 
-def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps: dict) -> dict:
+def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps: dict,
+                      diffusion_computed: float | None = None) -> dict:
     """Bytes each stage must move once per launch (every operand read or written
     exactly once), per the data layout of the library. `sweeps` holds the launches
     per step of the sweep stages. Sweeps count the cells a launch visits: all C
     cells for the first two pressure sweeps of a step (both pressure buffers are
     rewritten in full after the FLUID set changes) and for the first diffusion sweep
-    (it copies AIR cells across), the chunks holding FLUID otherwise."""
+    (it copies AIR cells across), the chunks holding FLUID otherwise. A paired sweep
+    moves diffusion bytes only when it computed its diffusion part
+    (`diffusion_computed` per step, the first sweep of a step always)."""
     X, Y, Z = cfg.size
     C = X * Y * Z
     Np = n_particles
@@ -68,6 +75,14 @@ def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps
 
     def per_launch(n, full):
         return (full * C + (n - full) * active_cells) / n if n else 0.0
+
+    def paired_diffusion(n, computed):
+        # cells whose diffusion a paired launch computes, on average: all C in the first
+        # sweep of a step, the FLUID chunks in the others that were not skipped
+        if not n:
+            return 0.0
+        computed = n if computed is None else max(1.0, min(float(n), computed))
+        return (C + (computed - 1) * active_cells) / n
 
     return {
         # G2P + RK3 + hash + bin count: position and velocity read, position, velocity
@@ -88,7 +103,8 @@ def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps
         "extrapolate": C * (12 + 1 + 12 + 1),  # per pass
         "diffusion": (1 + 8 * K) * per_launch(diff, 1 if diff and not paired else 0),
         # one pressure sweep and one diffusion sweep in the same launch
-        "pressure_diffusion": (13 + 1 + 8 * K) * per_launch(paired, min(2, paired)),
+        "pressure_diffusion": (13 * per_launch(paired, min(2, paired))
+                               + (1 + 8 * K) * paired_diffusion(paired, diffusion_computed)),
     }
 
 
@@ -131,6 +147,12 @@ def measured_traffic(scene: str, stage: str):
     if not per_kernel:
         return None
     return round(sum(b / n for b, n in per_kernel.values()))
+
+
+def sweeps_paired_possible(cfg, K: int) -> bool:
+    """Whether the step pairs pressure and diffusion sweeps (fixed-sweep solver, more
+    than one phase, both chains non-empty)."""
+    return K > 1 and cfg.pressure_iters > 0 and cfg.diffusion_iters > 0
 
 
 def load_warmstart_net():
@@ -229,6 +251,21 @@ def main() -> int:
     wall = t1 - t0
     ms_per_step = 1000.0 * wall / args.steps
 
+    # The same timed region with paired diffusion sweeps always computed (no skipping of
+    # sweeps that would recompute unchanged values), on a second handle of the same scene.
+    skip_off = None
+    if sweeps_paired_possible(cfg, K):
+        os.environ["AQFLIP_NO_FIXED_POINT_SKIP"] = "1"
+        try:
+            h2 = FlipHandle(cfg)
+        finally:
+            del os.environ["AQFLIP_NO_FIXED_POINT_SKIP"]
+        h2.steps(args.warmup)
+        t2 = time.perf_counter()
+        h2.steps(args.steps)
+        skip_off = round(args.steps / (time.perf_counter() - t2), 3)
+        h2.close()
+
     # Profiled pass: the same K steps launched eagerly with HIP events recorded on
     # the library's stream around every stage (read back once, at the end). The
     # library reports each stage's passes (sweeps, extrapolation passes, or runs),
@@ -236,6 +273,7 @@ def main() -> int:
     h.reset_timing()
     h.set_timing(True)
     active0 = swept_cells(h.get_field("content"), cfg.size)
+    computed0 = h.paired_diffusion_computed()
     tp0 = time.perf_counter()
     h.steps(args.steps)
     tp1 = time.perf_counter()
@@ -244,9 +282,10 @@ def main() -> int:
     eager_ms_per_step = 1000.0 * (tp1 - tp0) / args.steps
     # cells in chunks with FLUID, averaged over the start and the end of the pass
     active_cells = 0.5 * (active0 + swept_cells(h.get_field("content"), cfg.size))
+    diffusion_computed = (h.paired_diffusion_computed() - computed0) / args.steps
 
     sweeps = {name: launches / args.steps for name, (ms, launches) in timing.items()}
-    nbytes = algorithmic_bytes(cfg, n, K, active_cells, sweeps)
+    nbytes = algorithmic_bytes(cfg, n, K, active_cells, sweeps, diffusion_computed)
     kernels = {}
     for name, (ms, launches) in timing.items():
         if name not in nbytes or launches == 0:
@@ -323,6 +362,9 @@ def main() -> int:
         "roofline": roofline,
         "kernels": kernels,
         "sweep_cells": {"all": cfg.size[0] * cfg.size[1] * cfg.size[2], "visited_when_listed": round(active_cells)},
+        "paired_diffusion": {"sweeps_per_step": sweeps.get("pressure_diffusion", 0),
+                             "computed_per_step_profiled_pass": round(diffusion_computed, 2),
+                             "steps_per_s_skip_off": skip_off},
         "jacobi_to_tol": solve,
         "tolerance_mode": tol_mode,
         "profiled_pass_ms_per_step": round(eager_ms_per_step, 4),

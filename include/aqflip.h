@@ -221,6 +221,13 @@ aq_status aq_flip_set_timing(aq_flip* h, int32_t enable);
 aq_status aq_flip_get_timing(aq_flip* h, double* stage_ms, int64_t* stage_calls);
 aq_status aq_flip_reset_timing(aq_flip* h);
 
+/* Paired pressure + diffusion sweeps (AQ_STAGE_PRESSURE_DIFFUSION) whose diffusion
+ * part was computed, since the handle was created (32-bit device counter). A paired
+ * sweep skips its diffusion part when the previous diffusion sweep of the step changed
+ * no bit: every later sweep would recompute the same values from the same inputs.
+ * No counterpart in the reference, which always computes every sweep. */
+aq_status aq_flip_paired_diffusion_computed(aq_flip* h, int64_t* count);
+
 /* Simulated time (Fluid::physicalTime, Fluid/Fluid.h:11). */
 double aq_flip_physical_time(const aq_flip* h);
 /* Domain centre and size: Fluid_3D::getCenter / getContainerSize

@@ -236,6 +236,48 @@ def test_full_steps_match_oracle_multiphase_24(pressure_iters, diffusion_iters):
     gpu.close()
 
 
+@pytest.mark.parametrize("pressure_iters,diffusion_iters", [(40, 10), (12, 30)])
+def test_full_steps_match_oracle_mixing_24(pressure_iters, diffusion_iters):
+    """A two-phase ball resting in the pool: the phases mix from the first step, so
+    every paired diffusion sweep changes the fractions and none is skipped."""
+    cfg = configs.make_config(24, dt=0.01, pressure_iters=pressure_iters, diffusion_iters=diffusion_iters,
+                              radius=0.15, center=(0.5, 0.2, 0.5), sphere_phase=1, phase_count=2,
+                              diffusion_coeff=0.05, seed=23)
+    gpu, orc = pair(cfg)
+    log(f"[mixing 24 J{pressure_iters} D{diffusion_iters}] 4 steps")
+    gpu.steps(4)
+    orc.step(4)
+    check_particles(gpu, orc, "4 steps ")
+    check_cells(gpu, orc, "4 steps ")
+    for f in ("pressure", "new_velocity", "new_volume_fractions", "volume_fractions", "density"):
+        check_field(f, gpu, orc, "4 steps ")
+    frac = gpu.get_field("new_volume_fractions")[:24, :24, :24]
+    mixed = ((frac[..., 0] > 0) & (frac[..., 1] > 0)).sum()
+    assert mixed > 0, "the scene must mix phases"
+    gpu.close()
+
+
+@pytest.mark.parametrize("scene", ["separated", "mixing"])
+def test_fixed_point_skip_equals_full_compute_64(monkeypatch, scene):
+    """Paired diffusion sweeps after a sweep that changed nothing skip their work; the
+    steps give the same bits as computing every sweep."""
+    center = (0.5, 0.7, 0.5) if scene == "separated" else (0.5, 0.2, 0.5)
+    cfg = configs.make_config(64, dt=0.005, pressure_iters=60, diffusion_iters=40, radius=0.15,
+                              center=center, sphere_phase=1, phase_count=2, seed=29)
+    pos, frac = ob.seed(cfg)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("AQFLIP_NO_FIXED_POINT_SKIP", flag)
+        h = FlipHandle(cfg, pos=pos, frac4=frac)
+        log(f"[64 {scene} no_skip={flag}] 3 steps")
+        h.steps(3)
+        out[flag] = {f: h.get_field(f) for f in ("pressure", "new_velocity", "new_volume_fractions")}
+        out[flag]["pos"] = h.get_particles()[0]
+        h.close()
+    for k in out["0"]:
+        assert np.array_equal(out["0"][k].view(np.uint32), out["1"][k].view(np.uint32)), k
+
+
 def test_concurrent_diffusion_equals_sequential_128(monkeypatch):
     """Fused pressure + diffusion launches, diffusion on a second stream, and the
     reference's sequential order give the same bits at 128^3."""
