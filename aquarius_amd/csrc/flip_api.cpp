@@ -120,6 +120,16 @@ struct aq_flip {
   };
   std::vector<StepGraph> graphs;
   bool use_graphs = true;
+  // Tolerance solves: k Jacobi sweeps plus the FP64 residual kernels between two
+  // host checks, captured once per (pressure buffer, AIR-clean state, k).
+  struct SweepGraph {
+    int key[4];   // pb, air_clean[0], air_clean[1], k
+    int next[3];  // pb, air_clean[0], air_clean[1] afterwards
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<SweepGraph> sweep_graphs;
+  static constexpr size_t kMaxSweepGraphs = 16;
   // Multigrid-preconditioned CG (allocated on first use).
   struct MgState {
     MgHierarchy H;
@@ -137,6 +147,10 @@ struct aq_flip {
     if (stream) hipStreamSynchronize(stream);
     delete mg;
     for (StepGraph& g : graphs) {
+      hipGraphExecDestroy(g.exec);
+      hipGraphDestroy(g.graph);
+    }
+    for (SweepGraph& g : sweep_graphs) {
       hipGraphExecDestroy(g.exec);
       hipGraphDestroy(g.graph);
     }
@@ -572,14 +586,58 @@ aq_status mgpcg_solve(aq_flip* h, double tol, int max_iters, int check_every, bo
   return AQ_OK;
 }
 
-aq_status residual_now(aq_flip* h, double* rel) {
-  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
-  HIP_TRY(hipGetLastError());
+// Reads back the residual sums the residual kernels left in red_out.
+aq_status residual_read(aq_flip* h, double* rel) {
   double o[2];
   HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   *rel = o[1] > 0 ? std::sqrt(o[0]) / std::sqrt(o[1]) : std::sqrt(o[0]);
   return AQ_OK;
+}
+
+aq_status residual_now(aq_flip* h, double* rel) {
+  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  HIP_TRY(hipGetLastError());
+  return residual_read(h, rel);
+}
+
+// k Jacobi sweeps, then the FP64 residual of the result. Replayed from a captured
+// graph (host-launched sweeps cannot keep up with the GPU: about 5.6 against 3.6 us
+// per sweep at 128^3); eager while stage timing is on or graphs are switched off.
+aq_status sweeps_then_residual(aq_flip* h, int k, double* rel) {
+  if (h->timing || !h->use_graphs) {
+    jacobi_sweeps(h, k);
+    return residual_now(h, rel);
+  }
+  const int key[4] = {h->pb, h->air_clean[0], h->air_clean[1], k};
+  for (auto& g : h->sweep_graphs) {
+    if (std::memcmp(g.key, key, sizeof(key)) == 0) {
+      HIP_TRY(hipGraphLaunch(g.exec, h->stream));
+      h->pb = g.next[0], h->air_clean[0] = g.next[1] != 0, h->air_clean[1] = g.next[2] != 0;
+      return residual_read(h, rel);
+    }
+  }
+  if (h->sweep_graphs.size() >= aq_flip::kMaxSweepGraphs) {  // unusual chunk lengths: eager
+    jacobi_sweeps(h, k);
+    return residual_now(h, rel);
+  }
+  aq_flip::SweepGraph g{};
+  std::memcpy(g.key, key, sizeof(key));
+  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  jacobi_sweeps(h, k);
+  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  if (e != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    return fail(AQ_EHIP, std::string("sweep capture: ") + hipGetErrorString(e));
+  }
+  g.graph = graph;
+  HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+  g.next[0] = h->pb, g.next[1] = h->air_clean[0], g.next[2] = h->air_clean[1];
+  h->sweep_graphs.push_back(g);
+  HIP_TRY(hipGraphLaunch(g.exec, h->stream));
+  return residual_read(h, rel);
 }
 
 // Jacobi sweeps from the current pressure until the FP64 relative residual is
@@ -600,9 +658,8 @@ aq_status sweep_to_tol(aq_flip* h, int budget, bool detect_stall, int* sweeps, d
       break;
     }
     const int k = std::min(S.check_every, budget - *sweeps);
-    jacobi_sweeps(h, k);
+    AQ_TRY(sweeps_then_residual(h, k, res));
     *sweeps += k;
-    AQ_TRY(residual_now(h, res));
     if (detect_stall) {
       if (std::isfinite(*res) && *res < 0.99 * best) {
         best = *res;
@@ -1404,9 +1461,8 @@ aq_status aq_flip_solve_pressure(aq_flip* h, double tol, int32_t max_iters, int3
   AQ_TRY(aq_flip_residual_fp64(h, &rel));
   while (!(rel <= tol) && it < max_iters) {
     const int k = std::min(check_every, max_iters - it);
-    jacobi_sweeps(h, k);
+    AQ_TRY(sweeps_then_residual(h, k, &rel));
     it += k;
-    AQ_TRY(aq_flip_residual_fp64(h, &rel));
   }
   if (iters_out) *iters_out = it;
   if (rel_out) *rel_out = rel;

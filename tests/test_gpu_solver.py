@@ -23,6 +23,23 @@ def test_tolerance_solve_from_zero():
     h.close()
 
 
+def test_graph_replayed_tolerance_solves_match_eager(monkeypatch):
+    """The tolerance solve replays captured graphs of its sweep chunks (one per buffer
+    state and chunk length); every replay must give what eager launches give."""
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("AQFLIP_NO_GRAPH", flag)
+        h = FlipHandle(configs.ball_drop_64())
+        attach_solver(h, "tol_zero", tol=1e-4, max_iters=50000, check_every=30)
+        h.steps(3)
+        st = solver_stats(h)
+        it, rel, ok = h.solve_pressure(1e-5, 4000, 25)  # direct solves replay graphs too
+        out[flag] = (st["sweeps"], st["last_residual"], it, rel, h.get_field("pressure"))
+        h.close()
+    assert out["0"][:4] == out["1"][:4]
+    assert np.array_equal(out["0"][4].view(np.uint32), out["1"][4].view(np.uint32))
+
+
 def test_nan_network_falls_back_to_zero_guess():
     net = UNet(seed=2)
     net.set_params(np.full(net.param_count, np.nan, np.float32))
