@@ -174,8 +174,7 @@ void launch_advect(const Geom& G, Particles P, long long Np, Grid gr, int nv, St
 void launch_g2p_advect_hash(const Geom& G, Particles P, long long Np, int K, Grid gr, int nv,
                             int nf, StepParams sp, Bins B, hipStream_t s);
 // FP64 reductions: residual terms (sum r^2, sum b^2) and volume statistics.
-void launch_residual(const Geom& G, Grid gr, int pb, double* partial, int nparts, double* out,
-                     hipStream_t s);
+void launch_residual(const Geom& G, Grid gr, int pb, double* partial, double* out, hipStream_t s);
 void launch_stats(const Geom& G, Particles P, long long Np, int K, Grid gr, double* partial,
                   int nparts, double* out, hipStream_t s);
 

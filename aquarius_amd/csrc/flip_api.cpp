@@ -335,8 +335,9 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   AQ_TRY(h->allocT(&h->B.hash, (size_t)Np));
   AQ_TRY(h->allocT(&h->B.rank, (size_t)Np));
   AQ_TRY(h->allocT(&h->B.perm, (size_t)Np));
-  h->nparts_red = 512;
-  AQ_TRY(h->allocT(&h->red_partial, (size_t)h->nparts_red * 16));
+  h->nparts_red = 512;  // volume statistics: 512 partial rows of 16 columns
+  // (the residual uses two columns per sweep chunk)
+  AQ_TRY(h->allocT(&h->red_partial, std::max<size_t>((size_t)h->nparts_red * 16, 2 * (size_t)sweep_chunks(G))));
   AQ_TRY(h->allocT(&h->red_out, 16));
   return AQ_OK;
 }
@@ -596,7 +597,7 @@ aq_status residual_read(aq_flip* h, double* rel) {
 }
 
 aq_status residual_now(aq_flip* h, double* rel) {
-  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->red_out, h->stream);
   HIP_TRY(hipGetLastError());
   return residual_read(h, rel);
 }
@@ -625,7 +626,7 @@ aq_status sweeps_then_residual(aq_flip* h, int k, double* rel) {
   std::memcpy(g.key, key, sizeof(key));
   HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   jacobi_sweeps(h, k);
-  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->red_out, h->stream);
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(h->stream, &graph);
   if (e != hipSuccess) {
@@ -1426,7 +1427,7 @@ aq_status aq_flip_get_cells(aq_flip* h, int32_t* cell_start, int32_t* cell_end, 
 
 aq_status aq_flip_residual_fp64(aq_flip* h, double* rel) {
   if (!h || !rel) return fail(AQ_EINVAL, "null argument");
-  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->nparts_red, h->red_out, h->stream);
+  launch_residual(h->G, h->gr, h->pb, h->red_partial, h->red_out, h->stream);
   HIP_TRY(hipGetLastError());
   double o[2];
   HIP_TRY(hipMemcpyAsync(o, h->red_out, sizeof(o), hipMemcpyDeviceToHost, h->stream));

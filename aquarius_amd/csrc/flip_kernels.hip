@@ -1456,37 +1456,56 @@ __device__ double block_sum_d(double v) {
 // Residual of the pressure system over FLUID cells: A has diagonal
 // 6 - #(out-of-domain neighbours) and -1 per FLUID neighbour, AIR neighbours are
 // 0 (Fluid/Legacy/PressureEquation.cuh:216-328); b = -div*density.
+// Per chunk (kSweepSpan cells, two per thread, every load issued before use) the
+// sums of r^2 and b^2 over its FLUID cells, folded by a fixed-shape block tree into
+// partial[2*chunk + 0/1]; chunks without FLUID write zeros. The fold over the chunks
+// (k_fold_partials) runs in a fixed order, so the result is deterministic.
 __global__ void k_residual_partial(Geom G, const uint8_t* __restrict__ content,
                                    const float* __restrict__ rhs, const float* __restrict__ p,
-                                   double* __restrict__ partial) {
+                                   const uint32_t* __restrict__ active, double* __restrict__ partial) {
+  const int ch = xcd_block();
   double rr = 0.0, bb = 0.0;
-  const int stride = gridDim.x * blockDim.x;
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash; n += stride) {
-    int x, y, z;
-    cell_of(G, n, x, y, z);
-    const long long gi = G.g(x, y, z);
-    if (content[gi] != FLUID) continue;
-    const double b = (double)rhs[gi];
-    const double pc = (double)p[gi];
-    double ap = 6.0 * pc;
-    const bool in[6] = {x + 1 < G.X, x > 0, y + 1 < G.Y, y > 0, z + 1 < G.Z, z > 0};
-    const long long o[6] = {G.sx, -G.sx, (long long)G.EZ, -(long long)G.EZ, 1, -1};
+  if (chunk_active(active, ch)) {  // uniform per workgroup
+    SweepCell q[kSweepCells];
+    uint8_t c0[kSweepCells], cn[kSweepCells][6];
+    float bv[kSweepCells], pc[kSweepCells], pn[kSweepCells][6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      if (!in[k])
-        ap -= pc;
-      else if (content[gi + o[k]] == FLUID)
-        ap -= (double)p[gi + o[k]];
+    for (int c = 0; c < kSweepCells; ++c) {
+      q[c] = sweep_cell(G, ch, c);
+      c0[c] = content[q[c].gi];
+      bv[c] = rhs[q[c].gi];
+      pc[c] = p[q[c].gi];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        cn[c][k] = content[q[c].nb[k]];
+        pn[c][k] = p[q[c].nb[k]];
+      }
     }
-    const double r = b - ap;
-    rr += r * r;
-    bb += b * b;
+#pragma unroll
+    for (int c = 0; c < kSweepCells; ++c) {
+      if (!q[c].ok || c0[c] != FLUID) continue;
+      // A: diagonal 6 minus one per out-of-domain neighbour (a clamped neighbour index
+      // equals the cell's own), -1 per FLUID neighbour, AIR neighbours 0.
+      const double x = (double)pc[c];
+      double ap = 6.0 * x;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        if (q[c].nb[k] == q[c].gi)
+          ap -= x;
+        else if (cn[c][k] == FLUID)
+          ap -= (double)pn[c][k];
+      }
+      const double b = (double)bv[c];
+      const double r = b - ap;
+      rr += r * r;
+      bb += b * b;
+    }
   }
   rr = block_sum_d(rr);
   bb = block_sum_d(bb);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = rr;
-    partial[2 * blockIdx.x + 1] = bb;
+    partial[2 * ch] = rr;
+    partial[2 * ch + 1] = bb;
   }
 }
 
@@ -1509,11 +1528,11 @@ __global__ void k_fold_partials(const double* __restrict__ partial, int nparts, 
   }
 }
 
-void launch_residual(const Geom& G, Grid gr, int pb, double* partial, int nparts, double* out,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_residual_partial, dim3(nparts), dim3(kBlock), 0, s, G, gr.content, gr.rhs,
-                     gr.p[pb], partial);
-  hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(kBlock), 0, s, partial, nparts, 2, out);
+void launch_residual(const Geom& G, Grid gr, int pb, double* partial, double* out, hipStream_t s) {
+  const int nch = sweep_chunks(G);
+  hipLaunchKernelGGL(k_residual_partial, dim3(nch), dim3(kBlock), 0, s, G, gr.content, gr.rhs,
+                     gr.p[pb], gr.active, partial);
+  hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(kBlock), 0, s, partial, nch, 2, out);
 }
 
 // Volume and mass checks: per phase the sum of particle fractions, the number of

@@ -642,13 +642,33 @@ __global__ void k_nn_sigma_part(aqf::Geom G, const uint8_t* __restrict__ content
     part[2 * blockIdx.x + 1] = s2[0];
   }
 }
-__global__ void k_nn_sigma_fold(const double* __restrict__ part, int n, double* __restrict__ sig) {
-  if (threadIdx.x != 0) return;
-  double a = 0, c = 0;
-  for (int i = 0; i < n; ++i) {
-    a += part[2 * i];
-    c += part[2 * i + 1];
+// Sums of the two columns of n partial pairs, by one 256-thread block: thread t adds
+// pairs t, t + 256, ... in order, then a fixed-shape tree (deterministic). The result
+// is returned to thread 0.
+__device__ void fold_pairs(const double* __restrict__ part, int n, double& a, double& c) {
+  __shared__ double s1[256], s2[256];
+  double x = 0, y = 0;
+  for (int i = (int)threadIdx.x; i < n; i += 256) {
+    x += part[2 * i];
+    y += part[2 * i + 1];
   }
+  s1[threadIdx.x] = x;
+  s2[threadIdx.x] = y;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  a = s1[0];
+  c = s2[0];
+}
+__global__ void k_nn_sigma_fold(const double* __restrict__ part, int n, double* __restrict__ sig) {
+  double a, c;
+  fold_pairs(part, n, a, c);
+  if (threadIdx.x != 0) return;
   const double s = (c > 0) ? sqrt(a / c) : 0.0;
   sig[0] = s;
   sig[1] = s > 0 ? 1.0 / s : 0.0;
@@ -656,7 +676,7 @@ __global__ void k_nn_sigma_fold(const double* __restrict__ part, int n, double* 
 void nn_sigma(const aqf::Geom& G, const uint8_t* content, const float* rhs, double* part, int nparts,
               double* sig, hipStream_t s) {
   hipLaunchKernelGGL(k_nn_sigma_part, dim3(nparts), dim3(256), 0, s, G, content, rhs, part);
-  hipLaunchKernelGGL(k_nn_sigma_fold, dim3(1), dim3(64), 0, s, part, nparts, sig);
+  hipLaunchKernelGGL(k_nn_sigma_fold, dim3(1), dim3(256), 0, s, part, nparts, sig);
 }
 
 __global__ void k_nn_input(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ rhs,
@@ -872,12 +892,9 @@ __global__ void k_nn_alpha_part(aqf::Geom G, const uint8_t* __restrict__ content
   }
 }
 __global__ void k_nn_alpha_fold(const double* __restrict__ part, int n, double* __restrict__ alpha) {
+  double qb, qq;
+  fold_pairs(part, n, qb, qq);
   if (threadIdx.x != 0) return;
-  double qb = 0, qq = 0;
-  for (int i = 0; i < n; ++i) {
-    qb += part[2 * i];
-    qq += part[2 * i + 1];
-  }
   alpha[0] = (qq > 0 && isfinite(qb / qq)) ? qb / qq : 0.0;
 }
 __global__ void k_nn_alpha_apply(aqf::Geom G, const double* __restrict__ alpha, float* __restrict__ p) {
@@ -946,7 +963,7 @@ void nn_restrict(int Xc, int Yc, int Zc, const float* bf, const float* mf, float
 void nn_input_dense(long long V, const float* b, const float* m, double* part, int nparts, double* sig,
                     __bf16* x0, hipStream_t s) {
   hipLaunchKernelGGL(k_nn_sigma_dense, dim3(nparts), dim3(256), 0, s, V, b, m, part);
-  hipLaunchKernelGGL(k_nn_sigma_fold, dim3(1), dim3(64), 0, s, part, nparts, sig);
+  hipLaunchKernelGGL(k_nn_sigma_fold, dim3(1), dim3(256), 0, s, part, nparts, sig);
   hipLaunchKernelGGL(k_nn_input_dense, dim3(nblk(V)), dim3(256), 0, s, V, b, m, sig, x0);
 }
 void nn_output_dense(long long V, const float* y, const float* m, const double* sig, float scale, float* p,
@@ -960,7 +977,7 @@ void nn_scatter_scaled(const aqf::Geom& G, const uint8_t* content, const float* 
                        double* part, int nparts, double* alpha, hipStream_t s) {
   hipLaunchKernelGGL(k_nn_scatter, dim3(nblk(G.ncells_hash)), dim3(256), 0, s, G, content, pf, p);
   hipLaunchKernelGGL(k_nn_alpha_part, dim3(nparts), dim3(256), 0, s, G, content, rhs, p, part);
-  hipLaunchKernelGGL(k_nn_alpha_fold, dim3(1), dim3(64), 0, s, part, nparts, alpha);
+  hipLaunchKernelGGL(k_nn_alpha_fold, dim3(1), dim3(256), 0, s, part, nparts, alpha);
   hipLaunchKernelGGL(k_nn_alpha_apply, dim3(nblk(G.ncells_hash)), dim3(256), 0, s, G, alpha, p);
 }
 
