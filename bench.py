@@ -38,7 +38,15 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "FLIP steps/sec at 128^3, 8 ppc; Jacobi iters to 1e-4 residual, plain vs NN-warm"
-NN_REFRESH = 200  # sweeps between U-Net corrections of the iterate in the warm-start solves
+# Sweeps between U-Net corrections of the iterate in the warm-start solves. A correction
+# (inference through 64^3 and the energy-norm scaling) costs about as much as 280
+# sweeps at 128^3, so fewer corrections trade sweeps for time. Measured in tolerance
+# mode: every 100 / 150 / 200 / 300 / 400 / 600 sweeps gave 30.4 / 33.4 / 35.9 / 40.3 /
+# 40.2 / 41.0 steps/s with 2720 / 3110 / 3300 / 3410 / 3750 / 4150 sweeps per step.
+# The sweep-count comparison (jacobi_to_tol) corrects every 200 sweeps, the timed
+# tolerance-mode run every 300.
+NN_REFRESH = 200
+NN_REFRESH_TIMED = 300
 
 
 SWEEP_SPAN = 512  # cells per sweep chunk (the unit the listed sweeps skip or visit)
@@ -169,17 +177,17 @@ def load_warmstart_net():
 def tolerance_mode_steps(cfg, steps: int, cap: int):
     """Steps/s with every step's pressure solved to a 1e-4 FP64 relative residual
     (or until FP32 Jacobi stalls at its round-off floor): plain Jacobi from zero
-    against the U-Net warm start with a network correction every NN_REFRESH sweeps
+    against the U-Net warm start with a network correction every NN_REFRESH_TIMED sweeps
     (zero-guess fallback), on two fresh handles of the same workload."""
     from aquarius_amd.fluid import FlipHandle
     from aquarius_amd.unet import attach_solver, solver_stats
     net = load_warmstart_net()
     modes = [("plain", "tol_zero", None)] + ([("nn", "tol_nn", net)] if net is not None else [])
-    out = {"tol": 1e-4, "steps": steps, "check_every": 50}
+    out = {"tol": 1e-4, "steps": steps, "check_every": 50, "nn_refresh": NN_REFRESH_TIMED}
     for label, mode, nnet in modes:
         h = FlipHandle(cfg)
         attach_solver(h, mode, tol=1e-4, max_iters=cap, check_every=50, net=nnet,
-                      nn_refresh=NN_REFRESH if nnet is not None else 0)
+                      nn_refresh=NN_REFRESH_TIMED if nnet is not None else 0)
         h.steps(1)  # warm-up (first solve, lazy allocations)
         solver_stats(h, reset=True)
         t0 = time.perf_counter()
