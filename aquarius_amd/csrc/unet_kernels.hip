@@ -60,19 +60,25 @@ __device__ __forceinline__ long long vox_id(const Vol& v, int b, int d, int h, i
 constexpr int kConvMF = 4;
 constexpr int kConvVox = 4 * kConvMF * 16;  // voxels per 256-thread block
 
-template <int CIN, int COUT, bool RELU, bool OUT_F32, bool ACCUM>
+// MF fragments of 16 voxels per wave; the block computes NTW 16-channel output
+// tiles, tile group blockIdx.y (NTW = COUT/16: all of them). Small grids use MF = 1,
+// NTW = 1 so that they still spread over the CUs; each output value is the same
+// MFMA chain either way.
+template <int CIN, int COUT, bool RELU, bool OUT_F32, bool ACCUM, int MF = kConvMF, int NTW = COUT / 16>
 __global__ __launch_bounds__(256) void k_conv3(Vol v, const __bf16* __restrict__ X, int xs, int xo,
                                                const __bf16* __restrict__ Wk,
                                                const float* __restrict__ bias, void* __restrict__ Yv,
                                                int ys, int yo) {
   constexpr int K = 27 * CIN;
-  constexpr int NT = COUT / 16;
-  constexpr int MF = kConvMF;
+  constexpr int NT = NTW;
   constexpr int RS = K + 8;  // LDS row stride in bf16
-  constexpr bool WLDS = COUT * RS * 2 <= 64 * 1024;
-  __shared__ __align__(16) __bf16 sW[WLDS ? COUT * RS : 8];
+  constexpr bool WLDS = NT * 16 * RS * 2 <= 64 * 1024;
+  __shared__ __align__(16) __bf16 sW[WLDS ? NT * 16 * RS : 8];
+  const int n0 = (int)blockIdx.y * NT * 16;  // first output channel of this block
+  Wk += (long long)n0 * K;
+  if (bias) bias += n0;
   if (WLDS) {
-    for (int e = threadIdx.x; e < COUT * (K / 8); e += 256) {
+    for (int e = threadIdx.x; e < NT * 16 * (K / 8); e += 256) {
       const int n = e / (K / 8), j = e - n * (K / 8);
       *(bf16x8*)&sW[n * RS + j * 8] = *(const bf16x8*)(Wk + (long long)n * K + j * 8);
     }
@@ -128,7 +134,7 @@ __global__ __launch_bounds__(256) void k_conv3(Vol v, const __bf16* __restrict__
   for (int f = 0; f < MF; ++f)
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const int n = t * 16 + r;
+      const int n = t * 16 + r;  // within this block's tiles
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const long long o = wbase + f * 16 + q * 4 + i;
@@ -137,10 +143,10 @@ __global__ __launch_bounds__(256) void k_conv3(Vol v, const __bf16* __restrict__
         if (bias) val += bias[n];
         if (RELU) val = fmaxf(val, 0.f);
         if (OUT_F32) {
-          float* Y = (float*)Yv + o * ys + yo + n;
+          float* Y = (float*)Yv + o * ys + yo + n0 + n;
           *Y = ACCUM ? *Y + val : val;
         } else {
-          ((__bf16*)Yv)[o * ys + yo + n] = f2bf(val);
+          ((__bf16*)Yv)[o * ys + yo + n0 + n] = f2bf(val);
         }
       }
     }
@@ -150,11 +156,19 @@ template <int CIN, int COUT>
 static void conv3_fwd_t(const Vol& v, const __bf16* X, int xs, int xo, const __bf16* Wk,
                         const float* bias, __bf16* Y, int ys, int yo, bool relu, hipStream_t s) {
   const long long V = (long long)v.B * v.D * v.H * v.W;
-  const int nb = (int)((V + kConvVox - 1) / kConvVox);
-  if (relu)
-    hipLaunchKernelGGL((k_conv3<CIN, COUT, true, false, false>), dim3(nb), dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
-  else
-    hipLaunchKernelGGL((k_conv3<CIN, COUT, false, false, false>), dim3(nb), dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
+  if (V >= 1024 * (long long)kConvVox) {  // enough blocks: 4 fragments per wave, all tiles
+    const int nb = (int)((V + kConvVox - 1) / kConvVox);
+    if (relu)
+      hipLaunchKernelGGL((k_conv3<CIN, COUT, true, false, false>), dim3(nb), dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
+    else
+      hipLaunchKernelGGL((k_conv3<CIN, COUT, false, false, false>), dim3(nb), dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
+  } else {  // small grids: one fragment per wave, one 16-channel tile per block
+    const dim3 grid((unsigned)((V + 63) / 64), COUT / 16);
+    if (relu)
+      hipLaunchKernelGGL((k_conv3<CIN, COUT, true, false, false, 1, 1>), grid, dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
+    else
+      hipLaunchKernelGGL((k_conv3<CIN, COUT, false, false, false, 1, 1>), grid, dim3(256), 0, s, v, X, xs, xo, Wk, bias, (void*)Y, ys, yo);
+  }
 }
 template <int CIN, int COUT>
 static void conv3_dgrad_t(const Vol& v, const __bf16* dY, const __bf16* Wt, float* dX, int dxs, int dxo,
