@@ -265,7 +265,7 @@ def test_fixed_point_skip_equals_full_compute_64(monkeypatch, scene):
     cfg = configs.make_config(64, dt=0.005, pressure_iters=60, diffusion_iters=40, radius=0.15,
                               center=center, sphere_phase=1, phase_count=2, seed=29)
     pos, frac = ob.seed(cfg)
-    out = {}
+    out, computed = {}, {}
     for flag in ("1", "0"):
         monkeypatch.setenv("AQFLIP_NO_FIXED_POINT_SKIP", flag)
         h = FlipHandle(cfg, pos=pos, frac4=frac)
@@ -273,9 +273,16 @@ def test_fixed_point_skip_equals_full_compute_64(monkeypatch, scene):
         h.steps(3)
         out[flag] = {f: h.get_field(f) for f in ("pressure", "new_velocity", "new_volume_fractions")}
         out[flag]["pos"] = h.get_particles()[0]
+        computed[flag] = h.paired_diffusion_computed()
         h.close()
     for k in out["0"]:
         assert np.array_equal(out["0"][k].view(np.uint32), out["1"][k].view(np.uint32)), k
+    paired = 3 * min(cfg.pressure_iters, cfg.diffusion_iters)
+    assert computed["1"] == paired  # skipping off: every paired sweep computed
+    if scene == "separated":  # one-hot phases apart: only each step's first sweep computes
+        assert computed["0"] == 3
+    else:
+        assert 3 <= computed["0"] <= paired
 
 
 def test_concurrent_diffusion_equals_sequential_128(monkeypatch):
