@@ -1528,11 +1528,36 @@ __global__ void k_fold_partials(const double* __restrict__ partial, int nparts, 
   }
 }
 
+// Fold of the residual's per-chunk pairs: 1024 threads, thread t adds pairs t,
+// t + 1024, ... in order, then a fixed-shape tree (deterministic).
+__global__ __launch_bounds__(1024) void k_fold_pairs_1024(const double* __restrict__ partial, int n,
+                                                          double* __restrict__ out) {
+  __shared__ double s1[1024], s2[1024];
+  double a = 0.0, b = 0.0;
+  for (int i = (int)threadIdx.x; i < n; i += 1024) {
+    a += partial[2 * i];
+    b += partial[2 * i + 1];
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s1[threadIdx.x] += s1[threadIdx.x + o];
+      s2[threadIdx.x] += s2[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s1[0];
+    out[1] = s2[0];
+  }
+}
 void launch_residual(const Geom& G, Grid gr, int pb, double* partial, double* out, hipStream_t s) {
   const int nch = sweep_chunks(G);
   hipLaunchKernelGGL(k_residual_partial, dim3(nch), dim3(kBlock), 0, s, G, gr.content, gr.rhs,
                      gr.p[pb], gr.active, partial);
-  hipLaunchKernelGGL(k_fold_partials, dim3(1), dim3(kBlock), 0, s, partial, nch, 2, out);
+  hipLaunchKernelGGL(k_fold_pairs_1024, dim3(1), dim3(1024), 0, s, partial, nch, out);
 }
 
 // Volume and mass checks: per phase the sum of particle fractions, the number of
