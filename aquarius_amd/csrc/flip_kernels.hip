@@ -367,29 +367,35 @@ void launch_hash_outputs(const Geom& G, long long n, int stride, Bins B, const f
 // Fuses resetCellImpl (Fluid_3D_FLIP.cu:287-302), transferToCellAccumPhase (:8-69),
 // transferToCellDividePhase (:72-116) and transferVolumeFractionsToCell (:118-165).
 //
-// One wavefront per block owns a segment of 64 cells along z of one (x, y) row.
-// The reference visits, for each cell, its 27 neighbours in (dx, dy, dz) order by
-// flattened index h = (x+dx)*Y*Z + (y+dy)*Z + z+dz (range-checked in 1D only) and
-// each neighbour's particles in sorted order. For a fixed (dx, dy) the three
-// cells dz = -1, 0, 1 are consecutive hashes, so their particles form one
-// contiguous sorted range per lane, and the ranges of all 64 lanes form one
-// contiguous range per wave (row wrap-around included, exactly as the flattened
-// index wraps). The wave stages that range through LDS with coalesced 16-byte
-// loads; because every lane shares x and y, the x and y hat factors of a staged
-// particle are the same for the whole wave and are multiplied once, as
-// (kx * ky), the first product of trilinearHatKernel. Each lane then forms the
-// z factors and accumulates its range in order, so every sum sees the reference's
-// operands in the reference's order. Outside the centre slab (dx, dy) = (0, 0) about
-// half of the staged particles have x/y factor products that are all exactly zero;
-// they only add +-0 to sums that start at +0, which changes no bit for finite
-// velocities, so staging drops them (wave ballot, order kept) and each lane maps
-// its range onto the kept particles.
+// One wavefront per block owns a segment of kP2GCells cells along z of one (x, y)
+// row, two lanes per cell. The reference visits, for each cell, its 27 neighbours in
+// (dx, dy, dz) order by flattened index h = (x+dx)*Y*Z + (y+dy)*Z + z+dz
+// (range-checked in 1D only) and each neighbour's particles in sorted order. For a
+// fixed (dx, dy) the three cells dz = -1, 0, 1 are consecutive hashes, so their
+// particles form one contiguous sorted range per cell, and the ranges of the
+// segment's cells form one contiguous range per wave (row wrap-around included,
+// exactly as the flattened index wraps). The wave stages that range through LDS with
+// coalesced 16-byte loads, 256 particles per chunk (about the whole window of 28
+// cells, so every lane has work in every chunk); because every lane shares x and y,
+// the x and y hat factors of a staged particle are the same for the whole wave and
+// are multiplied once, as (kx * ky), the first product of trilinearHatKernel. Each
+// lane then forms its z factor and accumulates its cell's range in order, so every
+// sum sees the reference's operands in the reference's order. Outside the centre slab
+// (dx, dy) = (0, 0) about half of the staged particles have x/y factor products that
+// are all exactly zero; they only add +-0 to sums that start at +0, which changes no
+// bit for finite velocities, so staging drops them (wave ballot, order kept) and each
+// lane maps its range onto the kept particles.
+//
+// Lane 2i (role A) accumulates the u and v faces of cell i (weights with kzC) and the
+// cell-centre fractions; lane 2i + 1 (role B) the w face (weights with kzF). Both run
+// the same instructions on their own LDS records, A = (pu, pv, pz, vx) and
+// B = (pw, vy, pz, vz), both taking the second velocity from B.y = vy (role B's second
+// sum is meaningless and dropped). The centre slab stages every particle, so the
+// own-cell fractions are read from the particle record in memory. (One lane per cell
+// over 64-cell segments left half the lanes idle in each chunk: 441 against 391 us.)
 constexpr int kP2GWave = 64;
 constexpr int kP2GChunk = 256;  // particles staged per pass
-constexpr int kP2GSlots = kP2GChunk;
-// LDS slot of staged particle k. (A skewed slot k + k/8 against bank conflicts was
-// measured 2.5 % slower once the staging compacts away zero-weight particles.)
-__device__ __forceinline__ int p2g_slot(int k) { return k; }
+constexpr int kP2GCells = 28;   // cells per wave
 
 struct P2GGeom {
   int nsegz;
@@ -398,32 +404,31 @@ struct P2GGeom {
 
 template <int K>
 __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles P,
-                                                  const int* __restrict__ off, Grid gr, int nv,
-                                                  int hv, int nf, StepParams sp) {
-  __shared__ float4 sA[kP2GSlots];                  // (kxF*kyC, kxC*kyF, kxC*kyC, pz)
-  __shared__ float4 sB[kP2GSlots];                  // (vx, vy, vz, f0)
-  __shared__ float sF1[K == 2 ? kP2GSlots : 1];     // f1 (two phases)
-  __shared__ float4 sC[K > 2 ? kP2GSlots : 1];      // (f1, f2, f3, -) (three or four phases)
-  // block -> (x, y, z segment): b = (x*Y + y)*nsegz + zs
+                                                   const int* __restrict__ off, Grid gr, int nv,
+                                                   int hv, int nf, StepParams sp) {
+  __shared__ float4 sR[2][kP2GChunk];             // role records
   const int b = xcd_block();
   const int xy = (int)fast_div((uint32_t)b, PG.divSeg);
   const int zs = b - xy * PG.nsegz;
   const int x = (int)fast_div((uint32_t)xy, PG.divY);
   const int y = xy - x * G.Y;
   const int lane = threadIdx.x;
-  const int z0 = zs * kP2GWave;
-  const int z = z0 + lane;
-  const int zl = min(z0 + kP2GWave, G.Z) - 1;  // last cell of the segment
-  const bool active = z < G.Z;
+  const int ci = lane >> 1, role = lane & 1;
+  const int z0 = zs * kP2GCells;
+  const int z = z0 + ci;
+  const int zl = min(z0 + kP2GCells, G.Z) - 1;  // last cell of the segment
+  const bool active = ci < kP2GCells && z < G.Z;
   const float h = G.h;
   const double ih = G.inv_h;
   const float fx = (float)x, fy = (float)y, fz = (float)z;
   const float xF = fx * h, yF = fy * h, zF = fz * h;
   const float xC = (fx + 0.5f) * h, yC = (fy + 0.5f) * h, zC = (fz + 0.5f) * h;
+  const float zsel = role ? zF : zC;
+  const float4* __restrict__ myR = sR[role];
   const int NC = G.ncells_hash;
   const int YZ = G.Y * G.Z;
   const int own = x * YZ + y * G.Z + z;
-  float tv0 = 0.f, tv1 = 0.f, tv2 = 0.f, w0 = 0.f, w1 = 0.f, w2 = 0.f;
+  float t1 = 0.f, t2 = 0.f, s1 = 0.f, s2 = 0.f;
   float f[K];
 #pragma unroll
   for (int a = 0; a < K; ++a) f[a] = 0.f;
@@ -435,7 +440,7 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
   }
   for (int dx = -1; dx <= 1; ++dx)
     for (int dy = -1; dy <= 1; ++dy) {
-      const int h0 = (x + dx) * YZ + (y + dy) * G.Z;  // flattened index of (x+dx, y+dy, 0)
+      const int h0 = (x + dx) * YZ + (y + dy) * G.Z;
       const int lo = active ? off[clampi(h0 + z - 1, 0, NC)] : 0;
       const int hi = active ? off[clampi(h0 + z + 2, 0, NC)] : 0;
       const int wlo = off[clampi(h0 + z0 - 1, 0, NC)];
@@ -444,14 +449,12 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
       for (int cs = wlo; cs < whi; cs += kP2GChunk) {
         const int ce = min(cs + kP2GChunk, whi);
         const int nst = ce - cs;
-        __syncthreads();  // previous chunk fully consumed
-        // Stage the chunk; outside the centre slab keep only particles with a
-        // non-zero x/y factor product (the others add exact zeros to every sum).
+        __syncthreads();
         uint64_t kept[kP2GChunk / kP2GWave];
         int nkept = 0;
 #pragma unroll
         for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
-          if (bi * kP2GWave >= nst) {  // past the chunk's particles (wave-uniform)
+          if (bi * kP2GWave >= nst) {
             kept[bi] = 0;
             continue;
           }
@@ -473,16 +476,13 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
           const uint64_t m = __ballot(keep);
           kept[bi] = m;
           if (keep) {
-            const int q = p2g_slot(nkept + __popcll(m & ((1ull << lane) - 1)));
-            sA[q] = make_float4(pu, pv, pw, A.z);
-            sB[q] = make_float4(A.w, B.x, B.y, B.z);
-            if (K == 2) sF1[q] = B.w;
-            if (K > 2) sC[q] = make_float4(B.w, Cc.x, Cc.y, 0.f);
+            const int q = nkept + __popcll(m & ((1ull << lane) - 1));
+            sR[0][q] = make_float4(pu, pv, A.z, A.w);
+            sR[1][q] = make_float4(pw, B.x, A.z, B.y);
           }
           nkept += __popcll(m);
         }
         __syncthreads();
-        // This lane's range in staged (kept) order: kept particles before offset o.
         auto kept_before = [&](int o) {
           int c = 0;
 #pragma unroll
@@ -495,40 +495,33 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
         };
         const int c0 = kept_before(clampi(lo - cs, 0, nst));
         const int c1 = kept_before(clampi(hi - cs, 0, nst));
-        // One staged particle into this lane's sums; `own`: it lies in the lane's own
-        // cell (centre slab only), so it also feeds the cell-centre fractions.
-        auto accumulate = [&](int c, bool own) {
-          const int q = p2g_slot(c);
-          const float4 RA = sA[q];
-          const float4 RB = sB[q];
-          const float kzF = hat_div(RA.w - zF, ih), kzC = hat_div(RA.w - zC, ih);
-          const float wx = RA.x * kzC;  // u face (x, y+.5, z+.5)
-          const float wy = RA.y * kzC;  // v face (x+.5, y, z+.5)
-          const float wz = RA.z * kzF;  // w face (x+.5, y+.5, z)
-          tv0 += wx * RB.x;
-          tv1 += wy * RB.y;
-          tv2 += wz * RB.z;
-          w0 += wx;
-          w1 += wy;
-          w2 += wz;
-          if (own) {
-            const float wc = RA.z * kzC;  // cell centre (x+.5, y+.5, z+.5)
-            f[0] += wc * RB.w;
-            if (K == 2) f[1] += wc * sF1[q];
+        auto accumulate = [&](int c, bool ownc) {
+          const float4 R = myR[c];
+          const float vy = sR[1][c].y;
+          const float kz = hat_div(R.z - zsel, ih);
+          const float w1 = R.x * kz;  // A: u face (x, y+.5, z+.5); B: w face (x+.5, y+.5, z)
+          const float w2 = R.y * kz;  // A: v face (x+.5, y, z+.5); B: unused
+          t1 += w1 * R.w;
+          t2 += w2 * vy;
+          s1 += w1;
+          s2 += w2;
+          if (ownc) {  // role A: cell centre (x+.5, y+.5, z+.5), weight pw * kzC
+            const float wc = sR[1][c].x * kz;
+            const float4 Bq = P.b[cs + c];  // centre slab: slot c is particle cs + c
+            float fr[4] = {Bq.z, Bq.w, 0.f, 0.f};
             if (K > 2) {
-              const float4 RC = sC[q];
-              f[1] += wc * RC.x;
-              f[2] += wc * RC.y;
-              if (K > 3) f[3] += wc * RC.z;
+              const float4 Cq = P.c[cs + c];
+              fr[2] = Cq.x;
+              fr[3] = Cq.y;
             }
+#pragma unroll
+            for (int a = 0; a < K; ++a) f[a] += wc * fr[a];
             ws += wc;
           }
         };
         if (!centre) {
           for (int c = c0; c < c1; ++c) accumulate(c, false);
         } else {
-          // the centre slab keeps every particle, so c is its offset in the chunk; the
-          // lane's range splits into before / inside / after its own cell, in order
           const int o0 = clampi(ownLo - cs, c0, c1), o1 = clampi(ownHi - cs, o0, c1);
           for (int c = c0; c < o0; ++c) accumulate(c, false);
           for (int c = o0; c < o1; ++c) accumulate(c, true);
@@ -536,11 +529,14 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
         }
       }
     }
-  if (!active) return;
+  // role B's w-face results to its role-A partner
+  const float tw = __shfl_xor(t1, 1, 64), sw = __shfl_xor(s1, 1, 64);
+  if (!active || role) return;
+  float tv0 = t1, tv1 = t2, tv2 = tw;
   uint8_t hvb = 0;
-  if (w0 > 0.f) { tv0 /= w0; hvb |= 1; }
-  if (w1 > 0.f) { tv1 /= w1; hvb |= 2; }
-  if (w2 > 0.f) { tv2 /= w2; hvb |= 4; }
+  if (s1 > 0.f) { tv0 /= s1; hvb |= 1; }
+  if (s2 > 0.f) { tv1 /= s2; hvb |= 2; }
+  if (sw > 0.f) { tv2 /= sw; hvb |= 4; }
   const long long gi = G.g(x, y, z);
   gr.vel[0][gi] = tv0;
   gr.vel[1][gi] = tv1;
@@ -574,7 +570,7 @@ void launch_p2g(const Geom& G, Particles P, long long Np, int K, Bins B, Grid gr
                 int nf, StepParams sp, hipStream_t s) {
   (void)Np;
   P2GGeom PG;
-  PG.nsegz = (G.Z + kP2GWave - 1) / kP2GWave;
+  PG.nsegz = (G.Z + kP2GCells - 1) / kP2GCells;
   PG.divSeg = make_fastdiv((uint32_t)PG.nsegz);
   PG.divY = make_fastdiv((uint32_t)G.Y);
   const int nb = G.X * G.Y * PG.nsegz;
