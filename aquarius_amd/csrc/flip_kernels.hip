@@ -427,108 +427,166 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
   const float4* __restrict__ myR = sR[role];
   const int NC = G.ncells_hash;
   const int YZ = G.Y * G.Z;
-  const int own = x * YZ + y * G.Z + z;
   float t1 = 0.f, t2 = 0.f, s1 = 0.f, s2 = 0.f;
   float f[K];
 #pragma unroll
   for (int a = 0; a < K; ++a) f[a] = 0.f;
   float ws = 0.f;
-  int ownLo = 0, ownHi = 0;
-  if (active) {
-    ownLo = off[own];
-    ownHi = off[own + 1];
-  }
-  for (int dx = -1; dx <= 1; ++dx)
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int h0 = (x + dx) * YZ + (y + dy) * G.Z;
-      const int lo = active ? off[clampi(h0 + z - 1, 0, NC)] : 0;
-      const int hi = active ? off[clampi(h0 + z + 2, 0, NC)] : 0;
-      const int wlo = off[clampi(h0 + z0 - 1, 0, NC)];
-      const int whi = off[clampi(h0 + zl + 2, 0, NC)];
-      const bool centre = (dx == 0) & (dy == 0);
-      for (int cs = wlo; cs < whi; cs += kP2GChunk) {
-        const int ce = min(cs + kP2GChunk, whi);
-        const int nst = ce - cs;
-        __syncthreads();
-        uint64_t kept[kP2GChunk / kP2GWave];
-        int nkept = 0;
+  // Every bin offset the wave needs, in one round trip: slab s = 3 (dx + 1) + dy + 1
+  // holds off[clamp(h0 + z0 - 1 + j)], j = 0..31, in lanes 32 (s & 1) + j of wb[s >> 1]
+  // (h0 = (x + dx) Y Z + (y + dy) Z). Lane j = ci gives the cell's window start, ci + 3
+  // its end, 0 and zl - z0 + 3 the wave's; slab 4 at ci + 1, ci + 2 is the own cell.
+  int wb[5];
 #pragma unroll
-        for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
-          if (bi * kP2GWave >= nst) {
-            kept[bi] = 0;
-            continue;
-          }
-          const int k = bi * kP2GWave + lane;
-          bool keep = false;
-          float4 A = {}, B = {}, Cc = {};
-          float pu = 0.f, pv = 0.f, pw = 0.f;
-          if (k < nst) {
-            A = P.a[cs + k];
-            B = P.b[cs + k];
-            if (K > 2) Cc = P.c[cs + k];
-            const float kxF = hat_div(A.x - xF, ih), kxC = hat_div(A.x - xC, ih);
-            const float kyF = hat_div(A.y - yF, ih), kyC = hat_div(A.y - yC, ih);
-            pu = kxF * kyC;
-            pv = kxC * kyF;
-            pw = kxC * kyC;
-            keep = centre || pu != 0.f || pv != 0.f || pw != 0.f;
-          }
-          const uint64_t m = __ballot(keep);
-          kept[bi] = m;
-          if (keep) {
-            const int q = nkept + __popcll(m & ((1ull << lane) - 1));
-            sR[0][q] = make_float4(pu, pv, A.z, A.w);
-            sR[1][q] = make_float4(pw, B.x, A.z, B.y);
-          }
-          nkept += __popcll(m);
-        }
-        __syncthreads();
-        auto kept_before = [&](int o) {
-          int c = 0;
-#pragma unroll
-          for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
-            const int b0 = bi * kP2GWave;
-            if (o >= b0 + kP2GWave) c += __popcll(kept[bi]);
-            else if (o > b0) c += __popcll(kept[bi] & ((1ull << (o - b0)) - 1));
-          }
-          return c;
-        };
-        const int c0 = kept_before(clampi(lo - cs, 0, nst));
-        const int c1 = kept_before(clampi(hi - cs, 0, nst));
-        auto accumulate = [&](int c, bool ownc) {
-          const float4 R = myR[c];
-          const float vy = sR[1][c].y;
-          const float kz = hat_div(R.z - zsel, ih);
-          const float w1 = R.x * kz;  // A: u face (x, y+.5, z+.5); B: w face (x+.5, y+.5, z)
-          const float w2 = R.y * kz;  // A: v face (x+.5, y, z+.5); B: unused
-          t1 += w1 * R.w;
-          t2 += w2 * vy;
-          s1 += w1;
-          s2 += w2;
-          if (ownc) {  // role A: cell centre (x+.5, y+.5, z+.5), weight pw * kzC
-            const float wc = sR[1][c].x * kz;
-            const float4 Bq = P.b[cs + c];  // centre slab: slot c is particle cs + c
-            float fr[4] = {Bq.z, Bq.w, 0.f, 0.f};
-            if (K > 2) {
-              const float4 Cq = P.c[cs + c];
-              fr[2] = Cq.x;
-              fr[3] = Cq.y;
-            }
-#pragma unroll
-            for (int a = 0; a < K; ++a) f[a] += wc * fr[a];
-            ws += wc;
-          }
-        };
-        if (!centre) {
-          for (int c = c0; c < c1; ++c) accumulate(c, false);
-        } else {
-          const int o0 = clampi(ownLo - cs, c0, c1), o1 = clampi(ownHi - cs, o0, c1);
-          for (int c = c0; c < o0; ++c) accumulate(c, false);
-          for (int c = o0; c < o1; ++c) accumulate(c, true);
-          for (int c = o1; c < c1; ++c) accumulate(c, false);
-        }
-      }
+  for (int r = 0; r < 5; ++r) {
+    const int sl = 2 * r + (lane >> 5);
+    wb[r] = 0;
+    if (sl < 9) {
+      const int h0 = (x + sl / 3 - 1) * YZ + (y + sl % 3 - 1) * G.Z;
+      wb[r] = off[clampi(h0 + z0 - 1 + (lane & 31), 0, NC)];
     }
+  }
+  // (shuffles run on every lane: a lane outside the segment may be the source)
+  const int cj = min(ci, kP2GCells - 1);
+  int ownLo = __shfl(wb[2], cj + 1, 64), ownHi = __shfl(wb[2], cj + 2, 64);
+  if (!active) ownLo = ownHi = 0;
+  const int wend = zl - z0 + 3;
+  auto slab_offsets = [&](int sl) {  // wave-uniform sl
+    const int r = sl >> 1;
+    return r == 0 ? wb[0] : r == 1 ? wb[1] : r == 2 ? wb[2] : r == 3 ? wb[3] : wb[4];
+  };
+  auto win_lo = [&](int sl) { return __builtin_amdgcn_readlane(slab_offsets(sl), (sl & 1) << 5); };
+  auto win_hi = [&](int sl) { return __builtin_amdgcn_readlane(slab_offsets(sl), ((sl & 1) << 5) + wend); };
+  uint32_t busy = 1u << 9;  // slabs with particles in the window; bit 9 ends the walk
+#pragma unroll
+  for (int sl = 0; sl < 9; ++sl)
+    if (win_lo(sl) < win_hi(sl)) busy |= 1u << sl;
+  // The wave walks (slab, chunk) items in the reference's slab order. The records of the
+  // next item are loaded into registers while the current chunk is accumulated from LDS
+  // (after it, for the centre slab, whose accumulation reads fractions from memory).
+  float4 LA[kP2GChunk / kP2GWave];
+  float2 LB[kP2GChunk / kP2GWave];
+  // (unconditional loads, clamped to the item's last particle, so none waits in a branch)
+  auto load_item = [&](int sl, int cs) {
+    const int last = min(cs + kP2GChunk, win_hi(sl)) - 1;
+#pragma unroll
+    for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
+      const int i = min(cs + bi * kP2GWave + lane, last);
+      LA[bi] = P.a[i];
+      LB[bi] = reinterpret_cast<const float2*>(P.b)[2 * i];  // (vy, vz)
+    }
+  };
+  int sl = __builtin_ctz(busy);
+  int cs = sl < 9 ? win_lo(sl) : 0;
+  if (sl < 9) load_item(sl, cs);
+  while (sl < 9) {
+    const int base = (sl & 1) << 5;
+    const int v = slab_offsets(sl);
+    const int whi = __builtin_amdgcn_readlane(v, base + wend);
+    int lo = __shfl(v, base + cj, 64), hi = __shfl(v, base + cj + 3, 64);
+    if (!active) lo = hi = 0;
+    const bool centre = sl == 4;  // (dx, dy) = (0, 0)
+    const int ce = min(cs + kP2GChunk, whi);
+    const int nst = ce - cs;
+    int nsl = sl, ncs = ce;
+    if (ce >= whi) {
+      nsl = __builtin_ctz(busy & (~0u << (sl + 1)));
+      ncs = nsl < 9 ? win_lo(nsl) : 0;
+    }
+    __syncthreads();
+    uint64_t kept[kP2GChunk / kP2GWave];
+    int nkept = 0;
+#pragma unroll
+    for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
+      if (bi * kP2GWave >= nst) {
+        kept[bi] = 0;
+        continue;
+      }
+      const int k = bi * kP2GWave + lane;
+      bool keep = false;
+      const float4 A = LA[bi];
+      const float2 B = LB[bi];
+      float pu = 0.f, pv = 0.f, pw = 0.f;
+      if (k < nst) {
+        const float kxF = hat_div(A.x - xF, ih), kxC = hat_div(A.x - xC, ih);
+        const float kyF = hat_div(A.y - yF, ih), kyC = hat_div(A.y - yC, ih);
+        pu = kxF * kyC;
+        pv = kxC * kyF;
+        pw = kxC * kyC;
+        keep = centre || pu != 0.f || pv != 0.f || pw != 0.f;
+      }
+      const uint64_t m = __ballot(keep);
+      kept[bi] = m;
+      if (keep) {
+        // Narrow stores of the loaded registers as they are: a 16-byte store of values
+        // from two loads would copy them first, and a copy waits for the load.
+        const int q = nkept + __popcll(m & ((1ull << lane) - 1));
+        float* r0 = reinterpret_cast<float*>(&sR[0][q]);
+        float* r1 = reinterpret_cast<float*>(&sR[1][q]);
+        *reinterpret_cast<float2*>(r0) = make_float2(pu, pv);
+        asm volatile("" ::: "memory");
+        *reinterpret_cast<float2*>(r0 + 2) = make_float2(A.z, A.w);
+        asm volatile("" ::: "memory");
+        r1[0] = pw;
+        asm volatile("" ::: "memory");
+        r1[1] = B.x;
+        asm volatile("" ::: "memory");
+        r1[2] = A.z;
+        asm volatile("" ::: "memory");
+        r1[3] = B.y;
+      }
+      nkept += __popcll(m);
+    }
+    if (!centre && nsl < 9) load_item(nsl, ncs);
+    __syncthreads();
+    auto kept_before = [&](int o) {
+      int c = 0;
+#pragma unroll
+      for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
+        const int b0 = bi * kP2GWave;
+        if (o >= b0 + kP2GWave) c += __popcll(kept[bi]);
+        else if (o > b0) c += __popcll(kept[bi] & ((1ull << (o - b0)) - 1));
+      }
+      return c;
+    };
+    const int c0 = kept_before(clampi(lo - cs, 0, nst));
+    const int c1 = kept_before(clampi(hi - cs, 0, nst));
+    auto accumulate = [&](int c, bool ownc) {
+      const float4 R = myR[c];
+      const float vy = sR[1][c].y;
+      const float kz = hat_div(R.z - zsel, ih);
+      const float w1 = R.x * kz;  // A: u face (x, y+.5, z+.5); B: w face (x+.5, y+.5, z)
+      const float w2 = R.y * kz;  // A: v face (x+.5, y, z+.5); B: unused
+      t1 += w1 * R.w;
+      t2 += w2 * vy;
+      s1 += w1;
+      s2 += w2;
+      if (ownc) {  // role A: cell centre (x+.5, y+.5, z+.5), weight pw * kzC
+        const float wc = sR[1][c].x * kz;
+        const float4 Bq = P.b[cs + c];  // centre slab: slot c is particle cs + c
+        float fr[4] = {Bq.z, Bq.w, 0.f, 0.f};
+        if (K > 2) {
+          const float4 Cq = P.c[cs + c];
+          fr[2] = Cq.x;
+          fr[3] = Cq.y;
+        }
+#pragma unroll
+        for (int a = 0; a < K; ++a) f[a] += wc * fr[a];
+        ws += wc;
+      }
+    };
+    if (!centre) {
+      for (int c = c0; c < c1; ++c) accumulate(c, false);
+    } else {
+      const int o0 = clampi(ownLo - cs, c0, c1), o1 = clampi(ownHi - cs, o0, c1);
+      for (int c = c0; c < o0; ++c) accumulate(c, false);
+      for (int c = o0; c < o1; ++c) accumulate(c, true);
+      for (int c = o1; c < c1; ++c) accumulate(c, false);
+      if (nsl < 9) load_item(nsl, ncs);
+    }
+    sl = nsl;
+    cs = ncs;
+  }
   // role B's w-face results to its role-A partner
   const float tw = __shfl_xor(t1, 1, 64), sw = __shfl_xor(s1, 1, 64);
   if (!active || role) return;
