@@ -575,13 +575,22 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
         ws += wc;
       }
     };
+    // Slots [ca, cb) in order, two per trip (an odd count starts with one).
+    auto accumulate_range = [&](int ca, int cb) {
+      int c = ca;
+      if ((cb - ca) & 1) accumulate(c++, false);
+      for (; c < cb; c += 2) {
+        accumulate(c, false);
+        accumulate(c + 1, false);
+      }
+    };
     if (!centre) {
-      for (int c = c0; c < c1; ++c) accumulate(c, false);
+      accumulate_range(c0, c1);
     } else {
       const int o0 = clampi(ownLo - cs, c0, c1), o1 = clampi(ownHi - cs, o0, c1);
-      for (int c = c0; c < o0; ++c) accumulate(c, false);
+      accumulate_range(c0, o0);
       for (int c = o0; c < o1; ++c) accumulate(c, true);
-      for (int c = o1; c < c1; ++c) accumulate(c, false);
+      accumulate_range(o1, c1);
       if (nsl < 9) load_item(nsl, ncs);
     }
     sl = nsl;
