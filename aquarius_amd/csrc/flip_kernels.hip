@@ -575,13 +575,20 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
         ws += wc;
       }
     };
-    // Slots [ca, cb) in order, two per trip (an odd count starts with one).
+    // Slots [ca, cb) in order, four per trip (the count mod 4 first).
     auto accumulate_range = [&](int ca, int cb) {
       int c = ca;
       if ((cb - ca) & 1) accumulate(c++, false);
-      for (; c < cb; c += 2) {
+      if ((cb - ca) & 2) {
         accumulate(c, false);
         accumulate(c + 1, false);
+        c += 2;
+      }
+      for (; c < cb; c += 4) {
+        accumulate(c, false);
+        accumulate(c + 1, false);
+        accumulate(c + 2, false);
+        accumulate(c + 3, false);
       }
     };
     if (!centre) {
