@@ -86,6 +86,10 @@ struct Grid {
   uint32_t* active;     // per 256-cell launch block of the interior: any FLUID cell
                         // (32-bit so a block reads its flag with one scalar load)
   uint32_t* alist;      // [0] = count, then the ids of the sweep chunks holding FLUID
+  uint4* btab;          // per workgroup of a listed sweep grid: {chunks, first list
+                        // position, first two chunk ids}; the pressure grid's entries,
+                        // then the paired grid's (launch_active_list writes both)
+  int listed[2];        // workgroups of a listed pressure sweep and of a listed paired sweep
   uint32_t* dchange;    // words 0-2: paired diffusion sweep i changed a bit -> word i % 3 != 0;
                         // word 3: paired diffusion sweeps computed (not skipped)
 };
@@ -149,6 +153,11 @@ void launch_refresh_rhs(const Geom& G, Grid gr, hipStream_t s);
 void launch_active_list(const Geom& G, Grid gr, hipStream_t s);
 // Number of sweep chunks (the unit of gr.alist).
 int sweep_chunks(const Geom& G);
+// Entries of gr.btab (one per workgroup of both listed sweep grids, at their largest).
+int sweep_table_entries(const Geom& G);
+// gr.listed from the default grid sizes, or at most `limit` workgroups (limit > 0: a
+// diagnostic that makes each workgroup walk many list entries).
+void set_listed_grids(const Geom& G, Grid& gr, int limit);
 void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 // full = 1: every cell written (AIR cells get 0); full = 0: blocks without FLUID
 // cells and AIR cells are skipped -- valid once both buffers hold 0 on AIR.

@@ -80,6 +80,7 @@ struct aq_flip {
   bool fuse = true;
   // Paired diffusion sweeps after one that changed no bit skip their identical work.
   bool skip_fixed = true;
+  int listed_grid = 0;  // 0: the default listed sweep grids; n: at most n workgroups
   std::vector<void*> allocs;
 
   Particles P[2]{};
@@ -322,6 +323,8 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   // pairs of blocks never read past the array.
   AQ_TRY(h->allocT(&gr.active, (size_t)(G.ncells_hash + 255) / 256 + 1));
   AQ_TRY(h->allocT(&gr.alist, (size_t)sweep_chunks(G) + 1));
+  AQ_TRY(h->allocT(&gr.btab, (size_t)sweep_table_entries(G)));
+  set_listed_grids(G, gr, h->listed_grid);
   AQ_TRY(h->allocT(&gr.dchange, 4));
   for (int a = 0; a < 4; ++a) {
     AQ_TRY(h->allocT(&gr.vf[a], nc));
@@ -969,6 +972,7 @@ void read_env(aq_flip* h) {
   if (const char* env = std::getenv("AQFLIP_NO_OVERLAP")) h->overlap = env[0] != '1';
   if (const char* env = std::getenv("AQFLIP_NO_FUSE")) h->fuse = env[0] != '1';
   if (const char* env = std::getenv("AQFLIP_NO_FIXED_POINT_SKIP")) h->skip_fixed = env[0] != '1';
+  if (const char* env = std::getenv("AQFLIP_LISTED_GRID")) h->listed_grid = std::max(0, std::atoi(env));
 }
 
 aq_status create_common(const aq_flip_config* cfg, const float* pos, const float* vel,

@@ -828,9 +828,30 @@ __device__ __forceinline__ bool chunk_active(const uint32_t* __restrict__ active
 // Rebuilt after every update of the flags.
 constexpr int kListThreads = 1024;
 constexpr int kListBatch = 8;
+// Work split of a listed sweep over `total` list entries on `grid` workgroups: XCD k
+// (workgroups k, k + groups, ...) takes entries [k per, (k + 1) per), its workgroups
+// striding through them by nj.
+struct ListedSplit {
+  int groups, nj, per;
+};
+__host__ __device__ inline ListedSplit listed_split(int grid, int total) {
+  const int groups = grid >= kXCD ? kXCD : 1;
+  return ListedSplit{groups, grid / groups, (total + groups - 1) / groups};
+}
+// List position p is the k-th chunk of workgroup b (k = -1: none below 2, which is
+// all the table holds).
+__device__ __forceinline__ void listed_slot(const ListedSplit& L, int p, int& b, int& k) {
+  const int g = p / L.per, r = p - g * L.per;
+  const int j = r % L.nj;
+  k = r / L.nj;
+  b = j * L.groups + g;
+  if (k > 1) k = -1;
+}
 __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const uint32_t* __restrict__ active,
                                                               uint32_t* __restrict__ list,
-                                                              uint32_t* __restrict__ dchange) {
+                                                              uint32_t* __restrict__ dchange,
+                                                              uint4* __restrict__ btab, int gridJ,
+                                                              int gridP) {
   __shared__ int wsum[kListThreads / 64];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int per = (nchunks + kListThreads - 1) / kListThreads;
@@ -858,20 +879,52 @@ __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const
     total += wsum[w];
   }
   int pos = base + x - n;
+  // The sweeps' per-workgroup table: {chunks, first list position, first two ids}.
+  // Every id lands in the table slot its list position maps to, so nothing is read back.
+  const ListedSplit LJ = listed_split(gridJ, total), LP = listed_split(gridP, total);
   for (int b = c0; b < c1; b += kListBatch) {
     bool on[kListBatch];
 #pragma unroll
     for (int k = 0; k < kListBatch; ++k) on[k] = b + k < c1 && chunk_active(active, b + k);
 #pragma unroll
     for (int k = 0; k < kListBatch; ++k)
-      if (on[k]) list[1 + pos++] = (uint32_t)(b + k);
+      if (on[k]) {
+        const uint32_t id = (uint32_t)(b + k);
+        list[1 + pos] = id;
+        int wb, wk;
+        listed_slot(LJ, pos, wb, wk);
+        if (wk >= 0) reinterpret_cast<uint32_t*>(btab + wb)[2 + wk] = id;
+        listed_slot(LP, pos, wb, wk);
+        if (wk >= 0) reinterpret_cast<uint32_t*>(btab + gridJ + wb)[2 + wk] = id;
+        ++pos;
+      }
+  }
+  for (int e = t; e < gridJ + gridP; e += kListThreads) {
+    const bool pj = e < gridJ;
+    const ListedSplit& L = pj ? LJ : LP;
+    const int b = pj ? e : e - gridJ;
+    const int g = b % L.groups, j = b / L.groups;
+    const int t0 = g * L.per + j, end = min((g + 1) * L.per, total);
+    const int cnt = t0 < end ? (end - t0 + L.nj - 1) / L.nj : 0;
+    reinterpret_cast<uint32_t*>(btab + e)[0] = (uint32_t)cnt;
+    reinterpret_cast<uint32_t*>(btab + e)[1] = (uint32_t)t0;
   }
   if (t == 0) list[0] = (uint32_t)total;
   if (t == 0) dchange[0] = 0u;  // the first paired sweep of the next step records here
 }
+int sweep_grid(const Geom& G, bool full, int listed);
 void launch_active_list(const Geom& G, Grid gr, hipStream_t s) {
   hipLaunchKernelGGL(k_active_list, dim3(1), dim3(kListThreads), 0, s, sweep_chunks(G), gr.active, gr.alist,
-                     gr.dchange);
+                     gr.dchange, gr.btab, gr.listed[0], gr.listed[1]);
+}
+int sweep_table_entries(const Geom& G) {
+  return sweep_grid(G, false, kListedGrid) + sweep_grid(G, false, kListedGridPaired);
+}
+void set_listed_grids(const Geom& G, Grid& gr, int limit) {
+  const int lj = limit > 0 ? std::max(kXCD, std::min(limit, kListedGrid)) : kListedGrid;
+  const int lp = limit > 0 ? std::max(kXCD, std::min(limit, kListedGridPaired)) : kListedGridPaired;
+  gr.listed[0] = sweep_grid(G, false, lj);
+  gr.listed[1] = sweep_grid(G, false, lp);
 }
 
 // Work split of a sweep. A full sweep (every cell written) runs one workgroup per
@@ -885,18 +938,19 @@ int sweep_grid(const Geom& G, bool full, int listed = kListedGrid) {
   if (full || nch < kXCD) return nch;
   return std::min(nch, listed) / kXCD * kXCD;
 }
+// A listed workgroup reads its table entry (one scalar load, not dependent on the
+// list's length) and its chunks past the second from the list.
 template <typename F>
-__device__ __forceinline__ void for_each_chunk(const uint32_t* __restrict__ list, bool full, F&& body) {
+__device__ __forceinline__ void for_each_chunk(const uint4* __restrict__ tab, const uint32_t* __restrict__ list,
+                                               bool full, F&& body) {
   if (full) {
     body((int)xcd_block());
     return;
   }
-  const int cnt = (int)list[0];
-  const int groups = gridDim.x >= (unsigned)kXCD ? kXCD : 1;  // a listed grid is a multiple of kXCD
-  const int g = blockIdx.x % groups, j = blockIdx.x / groups, nj = gridDim.x / groups;
-  const int per = (cnt + groups - 1) / groups;
-  const int end = min((g + 1) * per, cnt);
-  for (int t = g * per + j; t < end; t += nj) body((int)list[1 + t]);
+  const uint4 e = tab[blockIdx.x];
+  const int n = (int)e.x;
+  const int nj = gridDim.x >= (unsigned)kXCD ? (int)gridDim.x / kXCD : (int)gridDim.x;
+  for (int k = 0; k < n; ++k) body((int)(k == 0 ? e.z : k == 1 ? e.w : list[1 + e.y + k * nj]));
 }
 
 // Cell c (0..kSweepCells-1) of this thread in chunk ch, its grid index and its six
@@ -962,17 +1016,18 @@ __device__ __forceinline__ void jacobi_chunk(const Geom& G, const float* __restr
 }
 template <bool PZERO>
 __global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
-                         float* __restrict__ pout, const uint32_t* __restrict__ list, int full) {
-  for_each_chunk(list, full != 0, [&](int ch) { jacobi_chunk<PZERO>(G, rhs, pin, pout, ch); });
+                         float* __restrict__ pout, const uint4* __restrict__ tab,
+                         const uint32_t* __restrict__ list, int full) {
+  for_each_chunk(tab, list, full != 0, [&](int ch) { jacobi_chunk<PZERO>(G, rhs, pin, pout, ch); });
 }
 void launch_jacobi(const Geom& G, Grid gr, int pin, int full, int pzero, hipStream_t s) {
-  const dim3 grid(sweep_grid(G, full != 0));
+  const dim3 grid(full || pzero ? sweep_grid(G, true) : gr.listed[0]);
   if (pzero)
     hipLaunchKernelGGL(k_jacobi<true>, grid, dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin], gr.p[pin ^ 1],
-                       gr.alist, 1);
+                       gr.btab, gr.alist, 1);
   else
     hipLaunchKernelGGL(k_jacobi<false>, grid, dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin], gr.p[pin ^ 1],
-                       gr.alist, full);
+                       gr.btab, gr.alist, full);
 }
 
 // diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
@@ -1268,8 +1323,8 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int 
 // the word sweep i + 1 will record into (word 0 is cleared by k_active_list, before
 // each step's first sweep). Word 3 counts the paired sweeps that computed diffusion.
 template <int K, bool PZERO>
-__global__ void k_jacobi_diffusion(Geom G, Grid gr, int pin, int fullJ, int fin, float lambda,
-                                   double inv_cc, int fullD, int sweep, int skip_ok) {
+__global__ void k_jacobi_diffusion(Geom G, Grid gr, const uint4* __restrict__ tab, int pin, int fullJ, int fin,
+                                   float lambda, double inv_cc, int fullD, int sweep, int skip_ok) {
   const bool skipD = skip_ok && !fullD && sweep > 0 && gr.dchange[(sweep - 1) % 3] == 0u;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     gr.dchange[(sweep + 1) % 3] = 0u;
@@ -1279,7 +1334,7 @@ __global__ void k_jacobi_diffusion(Geom G, Grid gr, int pin, int fullJ, int fin,
   // active list, both parts.
   const bool full = fullJ || fullD;
   bool changed = false;
-  for_each_chunk(gr.alist, full, [&](int ch) {
+  for_each_chunk(tab, gr.alist, full, [&](int ch) {
     const bool act = !full || chunk_active(gr.active, ch);
     const bool doJ = fullJ || act, doD = fullD || act;
     if (skipD) {  // the pressure sweep alone: no diffusion loads
@@ -1296,12 +1351,13 @@ __global__ void k_jacobi_diffusion(Geom G, Grid gr, int pin, int fullJ, int fin,
 template <int K>
 void launch_paired(const Geom& G, Grid gr, int pin, int fullJ, int fin, float lambda, double inv_cc,
                    int fullD, int pzero, int sweep, int skip_ok, hipStream_t s) {
-  const dim3 grid(sweep_grid(G, fullJ || fullD, kListedGridPaired));
+  const dim3 grid(fullJ || fullD ? sweep_grid(G, true) : gr.listed[1]);
+  const uint4* tab = gr.btab + gr.listed[0];  // the paired grid's entries
   if (pzero)
-    hipLaunchKernelGGL((k_jacobi_diffusion<K, true>), grid, dim3(kBlock), 0, s, G, gr, pin, fullJ, fin,
+    hipLaunchKernelGGL((k_jacobi_diffusion<K, true>), grid, dim3(kBlock), 0, s, G, gr, tab, pin, fullJ, fin,
                        lambda, inv_cc, fullD, sweep, skip_ok);
   else
-    hipLaunchKernelGGL((k_jacobi_diffusion<K, false>), grid, dim3(kBlock), 0, s, G, gr, pin, fullJ, fin,
+    hipLaunchKernelGGL((k_jacobi_diffusion<K, false>), grid, dim3(kBlock), 0, s, G, gr, tab, pin, fullJ, fin,
                        lambda, inv_cc, fullD, sweep, skip_ok);
 }
 void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,

@@ -285,6 +285,29 @@ def test_fixed_point_skip_equals_full_compute_64(monkeypatch, scene):
         assert 3 <= computed["0"] <= paired
 
 
+def test_listed_sweeps_on_eight_workgroups_64(monkeypatch):
+    """Listed sweeps read each workgroup's first two chunks from a table and the rest
+    from the chunk list. With 8 workgroups (AQFLIP_LISTED_GRID=8) each walks about 25
+    chunks, nearly all from the list; the steps give the same bits as the default grids
+    (at most two chunks per workgroup here)."""
+    cfg = configs.make_config(64, dt=0.005, pressure_iters=60, diffusion_iters=40, radius=0.15,
+                              center=(0.5, 0.2, 0.5), sphere_phase=1, phase_count=2, seed=29)
+    pos, frac = ob.seed(cfg)
+    out = {}
+    for grid in ("0", "8"):
+        monkeypatch.setenv("AQFLIP_LISTED_GRID", grid)
+        h = FlipHandle(cfg, pos=pos, frac4=frac)
+        log(f"[64 listed grid {grid}] 3 steps")
+        h.steps(3)
+        out[grid] = {f: h.get_field(f) for f in ("pressure", "new_velocity", "new_volume_fractions")}
+        out[grid]["pos"] = h.get_particles()[0]
+        active = int((h.get_field("content") == 1).sum())
+        h.close()
+    assert active > 8 * 512 * 2  # more FLUID cells than 8 workgroups' table entries cover
+    for k in out["0"]:
+        assert np.array_equal(out["0"][k].view(np.uint32), out["8"][k].view(np.uint32)), k
+
+
 def test_concurrent_diffusion_equals_sequential_128(monkeypatch):
     """Fused pressure + diffusion launches, diffusion on a second stream, and the
     reference's sequential order give the same bits at 128^3."""
