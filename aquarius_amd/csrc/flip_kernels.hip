@@ -184,28 +184,14 @@ __global__ void k_scan_reduce(const int* __restrict__ count, int n, int* __restr
   if (threadIdx.x == 0) partials[b] = tot;
 }
 
-__global__ void k_scan_partials(int* __restrict__ partials, int nparts, int* __restrict__ offsets,
-                                int n) {
-  const int per = (nparts + kBlock - 1) / kBlock;
-  const int b0 = threadIdx.x * per;
-  int s = 0;
-  for (int k = 0; k < per; ++k)
-    if (b0 + k < nparts) s += partials[b0 + k];
-  int tot;
-  int run = block_exclusive_scan(s, &tot);
-  for (int k = 0; k < per; ++k) {
-    if (b0 + k < nparts) {
-      const int v = partials[b0 + k];
-      partials[b0 + k] = run;
-      run += v;
-    }
-  }
-  if (threadIdx.x == 0) offsets[n] = tot;
-}
-
+// Block b adds the totals of blocks 0..b-1 (read here, with its counts, in the same
+// round trip) to its own exclusive scan; the last block also writes offsets[n].
+// Integer sums, so any association gives the same offsets.
 __global__ void k_scan_final(const int* __restrict__ count, int n, const int* __restrict__ partials,
-                             int* __restrict__ offsets) {
+                             int nparts, int* __restrict__ offsets) {
+  __shared__ int wsum[kBlock / 64], wpre[kBlock / 64];
   const int b = blockIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int base = b * kScanChunk + threadIdx.x * kScanPerThread;
   int v[kScanPerThread];
   int s = 0;
@@ -215,23 +201,40 @@ __global__ void k_scan_final(const int* __restrict__ count, int n, const int* __
     v[k] = (i < n) ? count[i] : 0;
     s += v[k];
   }
-  int tot;
-  int run = block_exclusive_scan(s, &tot) + partials[b];
+  int pre = 0;
+  for (int j = threadIdx.x; j < b; j += kBlock) pre += partials[j];
+  int x = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+    pre += __shfl_xor(pre, o, 64);
+  }
+  if (lane == 63) wsum[wid] = x;
+  if (lane == 0) wpre[wid] = pre;
+  __syncthreads();
+  int below = 0, tot = 0, prefix = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wid) below += wsum[w];
+    tot += wsum[w];
+    prefix += wpre[w];
+  }
+  int run = prefix + below + x - s;
 #pragma unroll
   for (int k = 0; k < kScanPerThread; ++k) {
     const int i = base + k;
     if (i < n) offsets[i] = run;
     run += v[k];
   }
+  if (b == nparts - 1 && threadIdx.x == 0) offsets[n] = prefix + tot;
 }
 
 void launch_scan(const Geom& G, Bins B, hipStream_t s) {
   const int n = G.ncells_hash;
   const int nparts = (n + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(kBlock), 0, s, B.count, n, B.partials);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, B.partials, nparts, B.offsets,
-                     n);
-  hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, s, B.count, n, B.partials,
+  hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, s, B.count, n, B.partials, nparts,
                      B.offsets);
 }
 

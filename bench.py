@@ -119,7 +119,7 @@ def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps
 # Kernels of each stage, as rocprofv3 names them (template arguments stripped).
 STAGE_KERNELS = {
     "g2p": ["aqf::k_g2p_advect_hash"],
-    "transfer_to_grid": ["aqf::k_scan_reduce", "aqf::k_scan_partials", "aqf::k_scan_final",
+    "transfer_to_grid": ["aqf::k_scan_reduce", "aqf::k_scan_final",
                          "aqf::k_scatter", "aqf::k_segsort", "aqf::k_gather"],
     "p2g": ["aqf::k_p2g"],
     "gravity": ["aqf::k_gravity_boundary"],
