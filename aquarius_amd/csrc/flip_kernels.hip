@@ -841,15 +841,6 @@ __host__ __device__ inline ListedSplit listed_split(int grid, int total) {
   const int groups = grid >= kXCD ? kXCD : 1;
   return ListedSplit{groups, grid / groups, (total + groups - 1) / groups};
 }
-// List position p is the k-th chunk of workgroup b (k = -1: none below 2, which is
-// all the table holds).
-__device__ __forceinline__ void listed_slot(const ListedSplit& L, int p, int& b, int& k) {
-  const int g = p / L.per, r = p - g * L.per;
-  const int j = r % L.nj;
-  k = r / L.nj;
-  b = j * L.groups + g;
-  if (k > 1) k = -1;
-}
 __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const uint32_t* __restrict__ active,
                                                               uint32_t* __restrict__ list,
                                                               uint32_t* __restrict__ dchange,
@@ -882,26 +873,19 @@ __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const
     total += wsum[w];
   }
   int pos = base + x - n;
-  // The sweeps' per-workgroup table: {chunks, first list position, first two ids}.
-  // Every id lands in the table slot its list position maps to, so nothing is read back.
-  const ListedSplit LJ = listed_split(gridJ, total), LP = listed_split(gridP, total);
   for (int b = c0; b < c1; b += kListBatch) {
     bool on[kListBatch];
 #pragma unroll
     for (int k = 0; k < kListBatch; ++k) on[k] = b + k < c1 && chunk_active(active, b + k);
 #pragma unroll
     for (int k = 0; k < kListBatch; ++k)
-      if (on[k]) {
-        const uint32_t id = (uint32_t)(b + k);
-        list[1 + pos] = id;
-        int wb, wk;
-        listed_slot(LJ, pos, wb, wk);
-        if (wk >= 0) reinterpret_cast<uint32_t*>(btab + wb)[2 + wk] = id;
-        listed_slot(LP, pos, wb, wk);
-        if (wk >= 0) reinterpret_cast<uint32_t*>(btab + gridJ + wb)[2 + wk] = id;
-        ++pos;
-      }
+      if (on[k]) list[1 + pos++] = (uint32_t)(b + k);
   }
+  if (t == 0) list[0] = (uint32_t)total;
+  // The sweeps' per-workgroup table: {chunks, first list position, first two ids},
+  // read back from the list this workgroup has just written.
+  __syncthreads();
+  const ListedSplit LJ = listed_split(gridJ, total), LP = listed_split(gridP, total);
   for (int e = t; e < gridJ + gridP; e += kListThreads) {
     const bool pj = e < gridJ;
     const ListedSplit& L = pj ? LJ : LP;
@@ -909,10 +893,9 @@ __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const
     const int g = b % L.groups, j = b / L.groups;
     const int t0 = g * L.per + j, end = min((g + 1) * L.per, total);
     const int cnt = t0 < end ? (end - t0 + L.nj - 1) / L.nj : 0;
-    reinterpret_cast<uint32_t*>(btab + e)[0] = (uint32_t)cnt;
-    reinterpret_cast<uint32_t*>(btab + e)[1] = (uint32_t)t0;
+    const uint32_t id0 = cnt > 0 ? list[1 + t0] : 0u, id1 = cnt > 1 ? list[1 + t0 + L.nj] : 0u;
+    btab[e] = make_uint4((uint32_t)cnt, (uint32_t)t0, id0, id1);
   }
-  if (t == 0) list[0] = (uint32_t)total;
   if (t == 0) dchange[0] = 0u;  // the first paired sweep of the next step records here
 }
 int sweep_grid(const Geom& G, bool full, int listed);
