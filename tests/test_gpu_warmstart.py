@@ -1,8 +1,8 @@
 """The trained warm start (aquarius_amd/weights/unet_warmstart_64.npz, produced by
 `python -m aquarius_amd.warmstart train`) against plain Jacobi from zero: both are
 refined to the same FP64 relative residual (1e-4), and the warm start must need
-fewer sweeps -- the gain the learned part is for. Plain Jacobi from zero is the
-judge of the error in both cases (the residual is measured, never predicted)."""
+fewer sweeps -- the gain the learned part is for. In both cases the measured FP64
+residual, never a prediction, decides when the solve has converged."""
 import numpy as np
 import pytest
 
