@@ -399,6 +399,12 @@ void launch_hash_outputs(const Geom& G, long long n, int stride, Bins B, const f
 // sum is meaningless and dropped). The centre slab stages every particle, so the
 // own-cell fractions are read from the particle record in memory. (One lane per cell
 // over 64-cell segments left half the lanes idle in each chunk: 441 against 391 us.)
+//
+// Latency: the bin offsets of all nine slabs arrive in one round trip per wave (not
+// one per slab), and the records of the next chunk are loaded into registers while
+// the current one is accumulated, so those loads overlap the accumulation. The kernel
+// is then bound by instruction issue; the hat is one clamped subtraction and the
+// accumulation loop takes four slots per trip.
 constexpr int kP2GWave = 64;
 constexpr int kP2GChunk = 256;  // particles staged per pass
 constexpr int kP2GCells = 28;   // cells per wave
