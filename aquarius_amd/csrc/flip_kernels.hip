@@ -173,15 +173,30 @@ __device__ int block_exclusive_scan(int v, int* total) {
   return base + x - v;
 }
 
+// A thread's kScanPerThread counts: 16-byte loads when all are in range (every
+// thread but the last one's), else one predicated load each.
+__device__ __forceinline__ void load_counts(const int* __restrict__ count, int n, int base,
+                                            int v[kScanPerThread]) {
+  if (base + kScanPerThread <= n) {
+    const int4* __restrict__ q = reinterpret_cast<const int4*>(count + base);
+#pragma unroll
+    for (int k = 0; k < kScanPerThread / 4; ++k) {
+      const int4 t = q[k];
+      v[4 * k] = t.x, v[4 * k + 1] = t.y, v[4 * k + 2] = t.z, v[4 * k + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) v[k] = base + k < n ? count[base + k] : 0;
+  }
+}
 __global__ void k_scan_reduce(const int* __restrict__ count, int n, int* __restrict__ partials) {
   const int b = blockIdx.x;
   const int base = b * kScanChunk + threadIdx.x * kScanPerThread;
+  int v[kScanPerThread];
+  load_counts(count, n, base, v);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanPerThread; ++k) {
-    const int i = base + k;
-    if (i < n) s += count[i];
-  }
+  for (int k = 0; k < kScanPerThread; ++k) s += v[k];
   int tot;
   block_exclusive_scan(s, &tot);
   if (threadIdx.x == 0) partials[b] = tot;
@@ -197,13 +212,10 @@ __global__ void k_scan_final(const int* __restrict__ count, int n, const int* __
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int base = b * kScanChunk + threadIdx.x * kScanPerThread;
   int v[kScanPerThread];
+  load_counts(count, n, base, v);
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanPerThread; ++k) {
-    const int i = base + k;
-    v[k] = (i < n) ? count[i] : 0;
-    s += v[k];
-  }
+  for (int k = 0; k < kScanPerThread; ++k) s += v[k];
   int pre = 0;
   for (int j = threadIdx.x; j < b; j += kBlock) pre += partials[j];
   int x = s;
@@ -224,11 +236,21 @@ __global__ void k_scan_final(const int* __restrict__ count, int n, const int* __
     prefix += wpre[w];
   }
   int run = prefix + below + x - s;
+  int o[kScanPerThread];
 #pragma unroll
   for (int k = 0; k < kScanPerThread; ++k) {
-    const int i = base + k;
-    if (i < n) offsets[i] = run;
+    o[k] = run;
     run += v[k];
+  }
+  if (base + kScanPerThread <= n) {
+    int4* __restrict__ q = reinterpret_cast<int4*>(offsets + base);
+#pragma unroll
+    for (int k = 0; k < kScanPerThread / 4; ++k)
+      q[k] = make_int4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k)
+      if (base + k < n) offsets[base + k] = o[k];
   }
   if (b == nparts - 1 && threadIdx.x == 0) offsets[n] = prefix + tot;
 }
