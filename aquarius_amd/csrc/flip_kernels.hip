@@ -872,6 +872,20 @@ __host__ __device__ inline ListedSplit listed_split(int grid, int total) {
   const int groups = grid >= kXCD ? kXCD : 1;
   return ListedSplit{groups, grid / groups, (total + groups - 1) / groups};
 }
+// Bit k: chunk b + k (< c1) holds FLUID. A chunk's kSweepCells = 2 flags are one
+// 8-byte load, unconditional (index clamped to the thread's last chunk) so the batch's
+// loads are in flight together.
+static_assert(kSweepCells == 2, "one 8-byte load per chunk's flags");
+__device__ __forceinline__ uint32_t batch_mask(const uint32_t* __restrict__ active, int b, int c1) {
+  uint2 f[kListBatch];
+#pragma unroll
+  for (int k = 0; k < kListBatch; ++k) f[k] = *reinterpret_cast<const uint2*>(active + 2 * min(b + k, c1 - 1));
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < kListBatch; ++k)
+    if (b + k < c1 && (f[k].x | f[k].y) != 0u) m |= 1u << k;
+  return m;
+}
 __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const uint32_t* __restrict__ active,
                                                               uint32_t* __restrict__ list,
                                                               uint32_t* __restrict__ dchange,
@@ -882,12 +896,11 @@ __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const
   const int per = (nchunks + kListThreads - 1) / kListThreads;
   const int c0 = min(t * per, nchunks), c1 = min(c0 + per, nchunks);
   int n = 0;
+  uint32_t first = 0;  // the first batch's mask, kept for the second pass
   for (int b = c0; b < c1; b += kListBatch) {
-    bool on[kListBatch];
-#pragma unroll
-    for (int k = 0; k < kListBatch; ++k) on[k] = b + k < c1 && chunk_active(active, b + k);
-#pragma unroll
-    for (int k = 0; k < kListBatch; ++k) n += on[k] ? 1 : 0;
+    const uint32_t m = batch_mask(active, b, c1);
+    if (b == c0) first = m;
+    n += __popc(m);
   }
   int x = n;  // inclusive scan within the wave, then across the waves
 #pragma unroll
@@ -905,12 +918,10 @@ __global__ __launch_bounds__(kListThreads) void k_active_list(int nchunks, const
   }
   int pos = base + x - n;
   for (int b = c0; b < c1; b += kListBatch) {
-    bool on[kListBatch];
-#pragma unroll
-    for (int k = 0; k < kListBatch; ++k) on[k] = b + k < c1 && chunk_active(active, b + k);
+    const uint32_t m = b == c0 ? first : batch_mask(active, b, c1);
 #pragma unroll
     for (int k = 0; k < kListBatch; ++k)
-      if (on[k]) list[1 + pos++] = (uint32_t)(b + k);
+      if (m & (1u << k)) list[1 + pos++] = (uint32_t)(b + k);
   }
   if (t == 0) list[0] = (uint32_t)total;
   // The sweeps' per-workgroup table: {chunks, first list position, first two ids},
