@@ -966,6 +966,13 @@ int sweep_grid(const Geom& G, bool full, int listed = kListedGrid) {
   if (full || nch < kXCD) return nch;
   return std::min(nch, listed) / kXCD * kXCD;
 }
+// A listed workgroup's chunks from its table entry e and, past the second, the list.
+template <typename F>
+__device__ __forceinline__ void for_each_listed(const uint4 e, const uint32_t* __restrict__ list, F&& body) {
+  const int n = (int)e.x;
+  const int nj = gridDim.x >= (unsigned)kXCD ? (int)gridDim.x / kXCD : (int)gridDim.x;
+  for (int k = 0; k < n; ++k) body((int)(k == 0 ? e.z : k == 1 ? e.w : list[1 + e.y + k * nj]));
+}
 // A listed workgroup reads its table entry (one scalar load, not dependent on the
 // list's length) and its chunks past the second from the list.
 template <typename F>
@@ -975,10 +982,7 @@ __device__ __forceinline__ void for_each_chunk(const uint4* __restrict__ tab, co
     body((int)xcd_block());
     return;
   }
-  const uint4 e = tab[blockIdx.x];
-  const int n = (int)e.x;
-  const int nj = gridDim.x >= (unsigned)kXCD ? (int)gridDim.x / kXCD : (int)gridDim.x;
-  for (int k = 0; k < n; ++k) body((int)(k == 0 ? e.z : k == 1 ? e.w : list[1 + e.y + k * nj]));
+  for_each_listed(tab[blockIdx.x], list, body);
 }
 
 // Cell c (0..kSweepCells-1) of this thread in chunk ch, its grid index and its six
@@ -1275,11 +1279,25 @@ void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, doub
 // shared cell and neighbour indices once. Every load of both is issued before any
 // is used, so the cells still cost one memory round trip. fullJ and fullD mean what
 // `full` means for each sweep.
+// The fields a paired sweep touches, passed as plain pointers: a kernel taking the whole
+// Grid by value read its fields from the argument segment one dependent round trip at a
+// time, after the branches that use them.
+struct PairedFields {
+  const float* rhs;
+  const float* pin;
+  float* pout;
+  const uint8_t* content;
+  const uint32_t* active;
+  const float* fin[4];
+  float* fout[4];
+};
 template <int K, bool PZERO>
-__device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int pin, int fin, float lambda,
+__device__ __forceinline__ void paired_chunk(const Geom& G, const PairedFields& gr, int pin, int fin, float lambda,
                                              double inv_cc, bool fullD, bool doJ, bool doD, int ch,
                                              bool& changed) {
-  const float* __restrict__ P = gr.p[pin];
+  (void)pin;
+  (void)fin;
+  const float* __restrict__ P = gr.pin;
   SweepCell q[kSweepCells];
   float b[kSweepCells], pv[kSweepCells][6], v[kSweepCells][7][K];
   uint8_t cc[kSweepCells];
@@ -1293,9 +1311,9 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int 
 #pragma unroll
     for (int k = 0; k < 6; ++k)
 #pragma unroll
-      for (int a = 0; a < K; ++a) v[c][k][a] = gr.nvf[fin][a][q[c].nb[k]];
+      for (int a = 0; a < K; ++a) v[c][k][a] = gr.fin[a][q[c].nb[k]];
 #pragma unroll
-    for (int a = 0; a < K; ++a) v[c][6][a] = gr.nvf[fin][a][q[c].gi];
+    for (int a = 0; a < K; ++a) v[c][6][a] = gr.fin[a][q[c].gi];
   }
 #pragma unroll
   for (int c = 0; c < kSweepCells; ++c) {
@@ -1310,12 +1328,12 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int 
   for (int c = 0; c < kSweepCells; ++c) {
     if (!q[c].ok) continue;
     const long long gi = q[c].gi;
-    if (doJ) gr.p[pin ^ 1][gi] = jacobi_value(b[c], pv[c]);
+    if (doJ) gr.pout[gi] = jacobi_value(b[c], pv[c]);
     if (!doD) continue;
     if (cc[c] == AIR) {  // diffusion_cell: AIR cells keep their value
       if (fullD) {
 #pragma unroll
-        for (int a = 0; a < K; ++a) gr.nvf[fin ^ 1][a][gi] = v[c][6][a];
+        for (int a = 0; a < K; ++a) gr.fout[a][gi] = v[c][6][a];
       }
       continue;
     }
@@ -1338,7 +1356,7 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int 
     for (int a = 0; a < K; ++a) {
       const float f = r[a] / sum;
       changed |= __float_as_uint(f) != __float_as_uint(v[c][6][a]);
-      gr.nvf[fin ^ 1][a][gi] = f;
+      gr.fout[a][gi] = f;
     }
   }
 }
@@ -1351,42 +1369,60 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const Grid& gr, int 
 // the word sweep i + 1 will record into (word 0 is cleared by k_active_list, before
 // each step's first sweep). Word 3 counts the paired sweeps that computed diffusion.
 template <int K, bool PZERO>
-__global__ void k_jacobi_diffusion(Geom G, Grid gr, const uint4* __restrict__ tab, int pin, int fullJ, int fin,
-                                   float lambda, double inv_cc, int fullD, int sweep, int skip_ok) {
-  const bool skipD = skip_ok && !fullD && sweep > 0 && gr.dchange[(sweep - 1) % 3] == 0u;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    gr.dchange[(sweep + 1) % 3] = 0u;
-    if (!skipD) atomicAdd(&gr.dchange[3], 1u);  // paired diffusion sweeps computed
-  }
+__global__ void k_jacobi_diffusion(Geom G, PairedFields f, const uint4* __restrict__ tab,
+                                   const uint32_t* __restrict__ list, uint32_t* __restrict__ dchange,
+                                   int fullJ, float lambda, double inv_cc, int fullD, int sweep, int skip_ok) {
   // Either sweep full: every chunk, each part where its own rule asks; otherwise the
-  // active list, both parts.
+  // active list, both parts. The previous sweep's change flag ((sweep + 2) % 3 is
+  // (sweep - 1) % 3 from sweep 1 on) and the table entry are loaded together.
   const bool full = fullJ || fullD;
+  const uint32_t prev = dchange[(sweep + 2) % 3];
+  const uint4 e = tab[full ? 0u : blockIdx.x];  // unconditional, so it issues with the flag
+  const bool skipD = skip_ok && !fullD && sweep > 0 && prev == 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    dchange[(sweep + 1) % 3] = 0u;
+    if (!skipD) atomicAdd(&dchange[3], 1u);  // paired diffusion sweeps computed
+  }
   bool changed = false;
-  for_each_chunk(tab, gr.alist, full, [&](int ch) {
-    const bool act = !full || chunk_active(gr.active, ch);
+  auto body = [&](int ch) {
+    const bool act = !full || chunk_active(f.active, ch);
     const bool doJ = fullJ || act, doD = fullD || act;
     if (skipD) {  // the pressure sweep alone: no diffusion loads
-      if (doJ) jacobi_chunk<PZERO>(G, gr.rhs, gr.p[pin], gr.p[pin ^ 1], ch);
+      if (doJ) jacobi_chunk<PZERO>(G, f.rhs, f.pin, f.pout, ch);
     } else if (doJ || doD) {
-      paired_chunk<K, PZERO>(G, gr, pin, fin, lambda, inv_cc, fullD != 0, doJ, doD, ch, changed);
+      paired_chunk<K, PZERO>(G, f, 0, 0, lambda, inv_cc, fullD != 0, doJ, doD, ch, changed);
     }
-  });
+  };
+  if (full)
+    body((int)xcd_block());
+  else
+    for_each_listed(e, list, body);
   // One plain store of 1 per wave that changed something: every writer stores the same
   // value, so the race is benign (a same-address atomic from every wave serialises: 39 us
   // per sweep in a mixing scene).
-  if (__ballot(changed) && (threadIdx.x & 63) == 0) gr.dchange[sweep % 3] = 1u;
+  if (__ballot(changed) && (threadIdx.x & 63) == 0) dchange[sweep % 3] = 1u;
 }
 template <int K>
 void launch_paired(const Geom& G, Grid gr, int pin, int fullJ, int fin, float lambda, double inv_cc,
                    int fullD, int pzero, int sweep, int skip_ok, hipStream_t s) {
   const dim3 grid(fullJ || fullD ? sweep_grid(G, true) : gr.listed[1]);
   const uint4* tab = gr.btab + gr.listed[0];  // the paired grid's entries
+  PairedFields f;
+  f.rhs = gr.rhs;
+  f.pin = gr.p[pin];
+  f.pout = gr.p[pin ^ 1];
+  f.content = gr.content;
+  f.active = gr.active;
+  for (int a = 0; a < 4; ++a) {
+    f.fin[a] = gr.nvf[fin][a];
+    f.fout[a] = gr.nvf[fin ^ 1][a];
+  }
   if (pzero)
-    hipLaunchKernelGGL((k_jacobi_diffusion<K, true>), grid, dim3(kBlock), 0, s, G, gr, tab, pin, fullJ, fin,
-                       lambda, inv_cc, fullD, sweep, skip_ok);
+    hipLaunchKernelGGL((k_jacobi_diffusion<K, true>), grid, dim3(kBlock), 0, s, G, f, tab, gr.alist, gr.dchange,
+                       fullJ, lambda, inv_cc, fullD, sweep, skip_ok);
   else
-    hipLaunchKernelGGL((k_jacobi_diffusion<K, false>), grid, dim3(kBlock), 0, s, G, gr, tab, pin, fullJ, fin,
-                       lambda, inv_cc, fullD, sweep, skip_ok);
+    hipLaunchKernelGGL((k_jacobi_diffusion<K, false>), grid, dim3(kBlock), 0, s, G, f, tab, gr.alist, gr.dchange,
+                       fullJ, lambda, inv_cc, fullD, sweep, skip_ok);
 }
 void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,
                              double inv_cc, int fullD, int pzero, int sweep, int skip_ok, hipStream_t s) {
