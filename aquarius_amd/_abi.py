@@ -175,6 +175,8 @@ PROTOTYPES = {
     "aq_unet_forward_host": (C.c_int, [_P, _FP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _FP]),
     "aq_unet_gradients_host": (C.c_int, [_P, _FP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          C.POINTER(C.c_double), _FP]),
+    "aq_unet_adam_step_host": (C.c_int, [_P, _FP]),
+    "aq_unet_get_adam_state": (C.c_int, [_P, _FP, _FP, C.POINTER(C.c_int64)]),
     "aq_unet_dataset_reserve": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "aq_unet_dataset_add_flip": (C.c_int, [_P, _P]),
     "aq_unet_dataset_size": (C.c_int32, [_P]),
@@ -192,8 +194,8 @@ PROTOTYPES = {
     # spatial hashing service (include/aqflip_hash.h)
     "aq_hash_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int64, C.POINTER(_P)]),
     "aq_hash_destroy": (None, [_P]),
-    "aq_hash_sort_host": (C.c_int, [_P, _FP, C.c_int64, C.c_int32, _IP, _IP, _IP, _IP, _FP]),
-    "aq_hash_sort_device": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P, _P]),
+    "aq_hash_sort_host": (C.c_int, [_P, _FP, C.c_int64, C.c_int32, _IP, _IP, _IP, _IP, C.c_int64, _FP]),
+    "aq_hash_sort_device": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P, C.c_int64, _P, _P]),
 }
 
 _lib = None

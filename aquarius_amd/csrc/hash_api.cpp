@@ -84,17 +84,18 @@ struct aq_hash {
     return AQ_OK;
   }
 
-  // The whole pipeline on device records; outputs are device pointers (may be null).
+  // The whole pipeline on device records, in order on stream s; outputs are device
+  // pointers (may be null).
   aq_status run(const float* rec, long long n, int stride, int* perm, int* shash, int* start, int* end,
-                float* sorted) {
+                float* sorted, hipStream_t s) {
     HTRY(ensure(n));
     const int ncells = G.ncells_hash;
-    launch_fill_u32(B.count, 0u, ncells, stream);
-    if (n > 0) launch_hash_records(G, rec, n, stride, B, stream);
-    launch_scan(G, B, stream);
-    if (n > 0) launch_scatter_segsort(G, n, B, stream);
-    launch_hash_outputs(G, n, stride, B, rec, start, end, shash, sorted, stream);
-    if (perm && n > 0) HHIP(hipMemcpyAsync(perm, B.perm, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
+    launch_fill_u32(B.count, 0u, ncells, s);
+    if (n > 0) launch_hash_records(G, rec, n, stride, B, s);
+    launch_scan(G, B, s);
+    if (n > 0) launch_scatter_segsort(G, n, B, s);
+    launch_hash_outputs(G, n, stride, B, rec, start, end, shash, sorted, s);
+    if (perm && n > 0) HHIP(hipMemcpyAsync(perm, B.perm, n * sizeof(int), hipMemcpyDeviceToDevice, s));
     HHIP(hipGetLastError());
     return AQ_OK;
   }
@@ -107,6 +108,15 @@ aq_status check_args(const aq_hash* h, const void* rec, long long n, int stride)
   if (n < 0 || n > 0x7fffffffLL) return hfail(AQ_EINVAL, "particle count must be in [0, 2^31)");
   if (n > 0 && !rec) return hfail(AQ_EINVAL, "null records");
   if (stride < 3) return hfail(AQ_EINVAL, "stride must be >= 3 floats");
+  return AQ_OK;
+}
+
+// cell_count: entries of the caller's cell_start / cell_end arrays; 0 means the grid's
+// cells. The reference clears cellCount entries (Fluid/Fluid_3D_common.cuh:47-48),
+// which the mesher sets above the hashed grid's cells (Rendering/Renderer3D/Mesher.cuh:155).
+aq_status check_cell_count(const aq_hash* h, long long cell_count) {
+  if (cell_count != 0 && cell_count < h->G.ncells_hash)
+    return hfail(AQ_EINVAL, "cell_count must be 0 or at least size_x*size_y*size_z");
   return AQ_OK;
 }
 
@@ -151,8 +161,9 @@ aq_status aq_hash_create(int32_t sx, int32_t sy, int32_t sz, float cell_size, in
 void aq_hash_destroy(aq_hash* h) { delete h; }
 
 aq_status aq_hash_sort_host(aq_hash* h, const float* rec, int64_t n, int32_t stride, int32_t* perm,
-                            int32_t* shash, int32_t* start, int32_t* end, float* sorted) {
+                            int32_t* shash, int32_t* start, int32_t* end, int64_t cell_count, float* sorted) {
   HTRY(check_args(h, rec, n, stride));
+  HTRY(check_cell_count(h, cell_count));
   for (long long i = 0; i < n; ++i)
     for (int a = 0; a < 3; ++a)
       if (!std::isfinite(rec[i * stride + a])) return hfail(AQ_EINVAL, "non-finite particle position");
@@ -160,23 +171,40 @@ aq_status aq_hash_sort_host(aq_hash* h, const float* rec, int64_t n, int32_t str
   const size_t bytes = (size_t)n * stride * sizeof(float);
   if (n > 0) HHIP(hipMemcpyAsync(h->d_rec, rec, bytes, hipMemcpyHostToDevice, h->stream));
   HTRY(h->run(h->d_rec, n, stride, nullptr, shash ? h->d_shash : nullptr, start ? h->d_start : nullptr,
-              end ? h->d_end : nullptr, sorted ? h->d_sorted : nullptr));
-  const size_t cb = (size_t)h->G.ncells_hash * sizeof(int);
+              end ? h->d_end : nullptr, sorted ? h->d_sorted : nullptr, h->stream));
+  const long long ncells = h->G.ncells_hash;
+  const size_t cb = (size_t)ncells * sizeof(int);
   if (perm && n > 0) HHIP(hipMemcpyAsync(perm, h->B.perm, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   if (shash && n > 0) HHIP(hipMemcpyAsync(shash, h->d_shash, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
   if (start) HHIP(hipMemcpyAsync(start, h->d_start, cb, hipMemcpyDeviceToHost, h->stream));
   if (end) HHIP(hipMemcpyAsync(end, h->d_end, cb, hipMemcpyDeviceToHost, h->stream));
   if (sorted && n > 0) HHIP(hipMemcpyAsync(sorted, h->d_sorted, bytes, hipMemcpyDeviceToHost, h->stream));
   HHIP(hipStreamSynchronize(h->stream));
+  if (cell_count > ncells) {
+    if (start) std::fill(start + ncells, start + cell_count, -1);
+    if (end) std::fill(end + ncells, end + cell_count, -1);
+  }
   return AQ_OK;
 }
 
 aq_status aq_hash_sort_device(aq_hash* h, const float* d_rec, int64_t n, int32_t stride, int32_t* d_perm,
-                              int32_t* d_shash, int32_t* d_start, int32_t* d_end, float* d_sorted) {
+                              int32_t* d_shash, int32_t* d_start, int32_t* d_end, int64_t cell_count,
+                              float* d_sorted, void* stream) {
   HTRY(check_args(h, d_rec, n, stride));
+  HTRY(check_cell_count(h, cell_count));
   if (d_sorted && d_sorted == d_rec) return hfail(AQ_EINVAL, "sorted records must not alias the input");
-  HTRY(h->run(d_rec, n, stride, d_perm, d_shash, d_start, d_end, d_sorted));
-  HHIP(hipStreamSynchronize(h->stream));
+  // The kernels run on the caller's stream, after the work already queued there (the
+  // reference's performSpatialHashing2 runs in the order of the default stream). The
+  // handle's own stream last ran a host-array call, which completed before it returned.
+  const hipStream_t s = (hipStream_t)stream;
+  HTRY(h->run(d_rec, n, stride, d_perm, d_shash, d_start, d_end, d_sorted, s));
+  const long long ncells = h->G.ncells_hash;
+  if (cell_count > ncells) {
+    if (d_start) launch_fill_u32(d_start + ncells, 0xffffffffu, cell_count - ncells, s);
+    if (d_end) launch_fill_u32(d_end + ncells, 0xffffffffu, cell_count - ncells, s);
+    HHIP(hipGetLastError());
+  }
+  HHIP(hipStreamSynchronize(s));
   return AQ_OK;
 }
 

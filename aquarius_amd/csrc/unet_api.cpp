@@ -428,6 +428,24 @@ aq_status aq_unet_set_params(aq_unet* net, const float* host) {
   return AQ_OK;
 }
 
+aq_status aq_unet_adam_step_host(aq_unet* net, const float* grads) {
+  if (!net || !grads) return ufail(AQ_EINVAL, "null argument");
+  UHIP(hipMemcpyAsync(net->grads, grads, net->nparam * sizeof(float), hipMemcpyHostToDevice, net->stream));
+  net->adam_step();
+  UHIP(hipGetLastError());
+  UHIP(hipStreamSynchronize(net->stream));
+  return AQ_OK;
+}
+
+aq_status aq_unet_get_adam_state(aq_unet* net, float* m, float* v, int64_t* t) {
+  if (!net) return ufail(AQ_EINVAL, "null argument");
+  if (m) UHIP(hipMemcpyAsync(m, net->m, net->nparam * sizeof(float), hipMemcpyDeviceToHost, net->stream));
+  if (v) UHIP(hipMemcpyAsync(v, net->v, net->nparam * sizeof(float), hipMemcpyDeviceToHost, net->stream));
+  UHIP(hipStreamSynchronize(net->stream));
+  if (t) *t = net->adam_t;
+  return AQ_OK;
+}
+
 aq_status aq_unet_forward_host(aq_unet* net, const float* x, int32_t B, int32_t D, int32_t H, int32_t W,
                                float* yh) {
   if (!net || !x || !yh) return ufail(AQ_EINVAL, "null argument");

@@ -89,6 +89,20 @@ class UNet:
                                               C.byref(loss), _fp(g)))
         return loss.value, g
 
+    def adam_step(self, grads) -> None:
+        """One Adam step of the FP32 master parameters with the given gradients."""
+        g = np.ascontiguousarray(grads, np.float32)
+        assert g.size == self.param_count
+        check(self.lib.aq_unet_adam_step_host(self.h, _fp(g)))
+
+    def adam_state(self):
+        """(m, v, t): the Adam moments and the step count."""
+        m = np.empty(self.param_count, np.float32)
+        v = np.empty(self.param_count, np.float32)
+        t = C.c_int64()
+        check(self.lib.aq_unet_get_adam_state(self.h, _fp(m), _fp(v), C.byref(t)))
+        return m, v, t.value
+
     # ---- device training set
     def reserve(self, capacity: int, shape) -> None:
         check(self.lib.aq_unet_dataset_reserve(self.h, capacity, *shape))

@@ -15,7 +15,8 @@
  * bounds there). Records are `stride` floats whose first three are the position:
  * stride 3 for packed positions, 12 for the reference's 48-byte Particle.
  *
- * Threading: one handle per thread; calls are synchronous at return.
+ * Threading: one handle per thread; calls are synchronous at return. Calls on one
+ * handle must not overlap (they share the handle's bins).
  */
 #ifndef AQFLIP_HASH_H
 #define AQFLIP_HASH_H
@@ -40,19 +41,30 @@ void aq_hash_destroy(aq_hash* h);
 /* Host arrays. records: n records of `stride` floats. Outputs (each may be NULL):
  *   perm[n]             sorted slot -> input index (the sorted particleIndices)
  *   sorted_hash[n]      the sorted particleHashes
- *   cell_start[cells], cell_end[cells]  first / last (inclusive) sorted slot of each
- *                       cell, -1 when the cell is empty (cells = size_x*size_y*size_z)
+ *   cell_start[cell_count], cell_end[cell_count]  first / last (inclusive) sorted slot
+ *                       of each cell, -1 when the cell is empty. cell_count is the
+ *                       length of these arrays: 0 means size_x*size_y*size_z; a larger
+ *                       count sets the entries past the grid's cells to -1, as the
+ *                       reference's clear of cellCount entries does
+ *                       (Fluid/Fluid_3D_common.cuh:47-48; the mesher passes an SDF cell
+ *                       count above the hashed grid's, Rendering/Renderer3D/Mesher.cuh:155)
  *   sorted_records[n*stride]  the records in sorted order
  * Positions must be finite. */
 aq_status aq_hash_sort_host(aq_hash* h, const float* records, int64_t n, int32_t stride,
                             int32_t* perm, int32_t* sorted_hash, int32_t* cell_start,
-                            int32_t* cell_end, float* sorted_records);
+                            int32_t* cell_end, int64_t cell_count, float* sorted_records);
 
-/* The same on device memory allocated by the caller (hipMalloc); complete at return.
- * A non-finite position hashes to a clamped cell instead of being rejected. */
+/* The same on device memory allocated by the caller (hipMalloc). stream is the
+ * caller's hipStream_t (NULL: the null stream): the kernels are queued on it, so they
+ * run after the work the caller queued there before the call (for instance the
+ * kernel that wrote the records), and the call returns once they are complete (it
+ * synchronises that stream, as the reference synchronises the device after its sort,
+ * Fluid/Fluid_3D_common.cuh:38). Work on other streams is not waited for. A
+ * non-finite position hashes to a clamped cell instead of being rejected. */
 aq_status aq_hash_sort_device(aq_hash* h, const float* d_records, int64_t n, int32_t stride,
                               int32_t* d_perm, int32_t* d_sorted_hash, int32_t* d_cell_start,
-                              int32_t* d_cell_end, float* d_sorted_records);
+                              int32_t* d_cell_end, int64_t cell_count, float* d_sorted_records,
+                              void* stream);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,14 @@ aq_status aq_unet_gradients_host(aq_unet* net, const float* x, const float* targ
                                  const float* mask, int32_t B, int32_t D, int32_t H, int32_t W,
                                  double* loss, float* grads);
 
+/* One Adam step (Kingma & Ba, bias-corrected through lr_t = lr*sqrt(1-b2^t)/(1-b1^t))
+ * on the FP32 master parameters with the given gradients (param_count floats), the
+ * update aq_unet_train applies after each batch's backward pass. */
+aq_status aq_unet_adam_step_host(aq_unet* net, const float* grads);
+/* The Adam moments m, v (param_count floats each; either may be NULL) and the step
+ * count t. */
+aq_status aq_unet_get_adam_state(aq_unet* net, float* m, float* v, int64_t* t);
+
 /* Training set on the device. */
 aq_status aq_unet_dataset_reserve(aq_unet* net, int32_t capacity, int32_t D, int32_t H, int32_t W);
 /* Append the current pressure system of a FLIP handle: its right-hand side and

@@ -458,3 +458,161 @@ def test_fp32_against_fp64_one_step():
         a, b = o32.get_field(name), o64.get_field(name)
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         assert rel < 1e-4, (name, rel)
+
+
+# ---------------------------------------------------------------- P2G fractions and density
+def brute_fractions(pos, frac, X, Y, Z, h, rho):
+    """transferVolumeFractionsToCell (Fluid/Fluid_3D_FLIP.cu:118-165) by brute force in
+    FP64: each cell averages the fractions of its OWN particles with trilinear weights
+    to the cell centre, renormalises them to sum 1 and takes density = sum rho_k f_k;
+    cells without particles get zero fractions and density 0."""
+    own = (pos.astype(np.float32) / np.float32(h)).astype(np.int64)  # the hash's truncation
+    f = np.zeros((X, Y, Z, 4))
+    for x in range(X):
+        for y in range(Y):
+            for z in range(Z):
+                sel = np.all(own == (x, y, z), axis=1)
+                if not sel.any():
+                    continue
+                r = (pos[sel].astype(np.float64) - (np.array([x, y, z]) + 0.5) * h) / h
+                w = hat(r[:, 0]) * hat(r[:, 1]) * hat(r[:, 2])
+                avg = (w[:, None] * frac[sel]).sum(0) / w.sum()
+                f[x, y, z] = avg / avg.sum()
+    return f, f @ np.asarray(rho, np.float64)
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (4, 3, 5)])
+def test_p2g_fractions_and_density_match_brute_force(shape):
+    X, Y, Z = shape
+    rho = (1.0, 2.5, 0.5, 4.0)
+    cfg = tiny_config(X, Y, Z, phase_count=4)
+    for a in range(4):
+        cfg.phase_densities[a] = rho[a]
+    h = cfg.grid_physical_size / Y
+    rng = np.random.default_rng(8)
+    n = 70
+    # positions kept 5 % away from cell walls so the own cell is unambiguous
+    cell = rng.integers(0, [X, Y, Z], size=(n, 3))
+    pos = ((cell + 0.05 + 0.9 * rng.random((n, 3))) * h).astype(np.float32)
+    frac = np.zeros((n, 4), np.float32)
+    onehot = rng.random(n) < 0.5
+    frac[onehot, rng.integers(0, 4, onehot.sum())] = 1.0
+    # fractional rows, deliberately not summing to 1 (the cell result is renormalised)
+    frac[~onehot] = rng.random(((~onehot).sum(), 4)).astype(np.float32) * 0.8
+    o = ob.Oracle(cfg, fp64=True, pos=pos, frac4=frac, init=False)
+    o.run_stage("transfer_to_grid")
+    want_f, want_rho = brute_fractions(pos, frac.astype(np.float64), X, Y, Z, h, rho)
+    for name in ("volume_fractions", "new_volume_fractions"):
+        np.testing.assert_allclose(o.get_field(name), want_f, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(o.get_field("density"), want_rho, rtol=1e-12, atol=1e-14)
+    assert (want_rho == 0).any() and (want_rho > 0).any()  # empty and occupied cells both present
+
+
+def test_p2g_fractions_two_particle_hand_case():
+    """One particle at the cell centre (weight 1, phase 0) and one offset by h/4 on every
+    axis (weight 0.75^3 = 0.421875, phase 1): f = (1, 0.421875) / 1.421875 and
+    density = (1*1 + 2.5*0.421875) / 1.421875 with phase densities (1, 2.5)."""
+    cfg = tiny_config(2, 2, 2, phase_count=2)
+    cfg.phase_densities[0], cfg.phase_densities[1] = 1.0, 2.5
+    h = cfg.grid_physical_size / 2  # 5.0
+    pos = np.array([[1.5 * h, 0.5 * h, 0.5 * h], [1.75 * h, 0.75 * h, 0.75 * h]], np.float32)
+    frac = np.array([[1, 0, 0, 0], [0, 1, 0, 0]], np.float32)
+    o = ob.Oracle(cfg, fp64=True, pos=pos, frac4=frac, init=False)
+    o.run_stage("transfer_to_grid")
+    f = o.get_field("volume_fractions")
+    np.testing.assert_allclose(f[1, 0, 0], [1 / 1.421875, 0.421875 / 1.421875, 0, 0], rtol=1e-15)
+    assert o.get_field("density")[1, 0, 0] == pytest.approx(2.0546875 / 1.421875, rel=1e-15)
+    # every other cell is empty: zero fractions, density 0
+    d = o.get_field("density")
+    d[1, 0, 0] = 0
+    assert np.all(d == 0) and np.all(f.reshape(-1, 4)[[0, 1, 2, 3, 5, 6, 7]] == 0)
+
+
+# ---------------------------------------------------------------- projection
+def test_projection_density_branches_hand_case():
+    """updateVelocityWithPressureImpl (Fluid/Fluid_3D_kernels.cu:118-207) on a hand-built
+    3^3 case. Cells (x, y, z) with (content, density, pressure):
+      (0,1,1) FLUID 2 3;  (1,1,1) FLUID 4 1;  (2,1,1) AIR 0 0;
+      (1,0,1) SOLID 7 5;  (0,1,0) AIR 9 0.5;  (2,0,1) AIR 0 0.
+    Expected face updates (u -= (p - p_lower_neighbour) / rho_used):
+      u(1,1,1): FLUID|FLUID, rho (2+4)/2 = 3 -> u + 2/3
+      u(2,1,1): AIR|FLUID(left), rho_left = 4 -> u + 1/4
+      v(1,1,1): FLUID over SOLID, rho_this = 4 -> v + 1      (the SOLID's pressure is read)
+      w(0,1,1): FLUID over AIR, rho_this = 2 -> w - 5/4       (the AIR's density is ignored)
+      v(2,1,1): AIR over AIR -> unchanged, hasVelocity 0
+    hasVelocity is overwritten on every cell with 'face touches FLUID'."""
+    X = Y = Z = 3
+    cfg = tiny_config(X, Y, Z)
+    o = ob.Oracle(cfg, fp64=True, init=False)
+    content = np.full((4, 4, 4), AIR, np.int32)
+    content[3, :, :] = SOLID
+    content[:, 3, :] = SOLID
+    content[:, :, 3] = SOLID
+    dens = np.zeros((3, 3, 3))
+    p = np.zeros((3, 3, 3))
+    for c, (k, d, pr) in {(0, 1, 1): (FLUID, 2, 3), (1, 1, 1): (FLUID, 4, 1), (2, 1, 1): (AIR, 0, 0),
+                          (1, 0, 1): (SOLID, 7, 5), (0, 1, 0): (AIR, 9, 0.5),
+                          (2, 0, 1): (AIR, 0, 0)}.items():
+        content[c], dens[c], p[c] = k, d, pr
+    rng = np.random.default_rng(12)
+    nv = rng.standard_normal((4, 4, 4, 4))
+    o.set_field("content", content)
+    o.set_field("density", dens)
+    o.set_field("pressure", p)
+    o.set_field("new_velocity", nv)
+    o.set_field("has_velocity", np.ones((3, 3, 3, 4)))
+    o.run_stage("project")
+    got = o.get_field("new_velocity")
+    assert got[1, 1, 1, 0] == pytest.approx(nv[1, 1, 1, 0] + 2 / 3, rel=1e-15)
+    assert got[2, 1, 1, 0] == pytest.approx(nv[2, 1, 1, 0] + 0.25, rel=1e-15)
+    assert got[1, 1, 1, 1] == pytest.approx(nv[1, 1, 1, 1] + 1.0, rel=1e-15)
+    assert got[0, 1, 1, 2] == pytest.approx(nv[0, 1, 1, 2] - 1.25, rel=1e-15)
+    assert got[2, 1, 1, 1] == nv[2, 1, 1, 1]
+    # faces not touching FLUID keep their value; the 4th channel is untouched
+    fl = content[:3, :3, :3] == FLUID
+    touch = np.zeros((3, 3, 3, 3), bool)
+    touch[1:, :, :, 0] = fl[1:] | fl[:-1]
+    touch[:, 1:, :, 1] = fl[:, 1:] | fl[:, :-1]
+    touch[:, :, 1:, 2] = fl[:, :, 1:] | fl[:, :, :-1]
+    hv = o.get_field("has_velocity")
+    assert np.array_equal(hv[..., :3].astype(bool), touch) and np.all(hv[..., 3] == 0)
+    assert touch.sum() == 10  # 6 faces of (1,1,1), 4 of (0,1,1) (its x face is at x = 0)
+    assert np.array_equal(got[:3, :3, :3, :3][~touch], nv[:3, :3, :3, :3][~touch])
+    assert np.array_equal(got[..., 3], nv[..., 3])
+    # the face-extent padding layer (index 3) is outside the launch and unchanged
+    assert np.array_equal(got[3], nv[3]) and np.array_equal(got[:, 3], nv[:, 3])
+
+
+# ---------------------------------------------------------------- G2P fraction read
+def test_g2p_reads_fractions_of_the_particles_own_cell():
+    """transferToParticlesImpl (Fluid/Fluid_3D_FLIP.cu:208-224): a particle takes
+    newVolumeFractions of cell floor(p/h) -- no interpolation -- renormalised to sum 1
+    when the sum is positive. The field varies from cell to cell, so an interpolated or
+    rounded read would give other values; the old volumeFractions field is not read."""
+    X = Y = Z = 4
+    cfg = tiny_config(X, Y, Z, phase_count=2)
+    h = cfg.grid_physical_size / Y  # 2.5, exact
+    cells = np.array([[1, 2, 1], [1, 2, 1], [2, 0, 3], [3, 3, 0], [0, 1, 2], [2, 2, 2]])
+    offs = np.array([[0.9, 0.9, 0.9], [0.1, 0.5, 0.05], [0.5, 0.95, 0.5], [0.99, 0.01, 0.6],
+                     [0.0, 0.0, 0.0], [0.3, 0.7, 0.2]])  # (0,1,2)+0 lies exactly on cell walls
+    pos = ((cells + offs) * h).astype(np.float32)
+    assert np.array_equal(np.floor(pos.astype(np.float64) / h).astype(int), cells)
+    nvf = np.zeros((X, Y, Z, 4))
+    idx = np.arange(X * Y * Z).reshape(X, Y, Z)
+    nvf[..., 0] = idx + 1.0
+    nvf[..., 1] = 2.0
+    nvf[3, 3, 0] = 0.0  # a zero-sum cell: fractions copied unnormalised (all zero)
+    o = ob.Oracle(cfg, fp64=True, pos=pos, init=False)
+    o.set_field("new_volume_fractions", nvf)
+    o.set_field("volume_fractions", np.full((X, Y, Z, 4), 0.25))
+    o.run_stage("g2p")
+    _, _, f = o.get_particles()
+    # the oracle may keep particles in another order than given: match by position
+    gp, _, _ = o.get_particles()
+    order = [int(np.argmin(np.abs(gp - q).sum(1))) for q in pos.astype(np.float64)]
+    for k, c in enumerate(map(tuple, cells)):
+        s = nvf[c].sum()
+        want = nvf[c] / s if s > 0 else nvf[c]
+        np.testing.assert_allclose(f[order[k]], want, rtol=1e-15, atol=0)
+    assert np.all(f[order[3]] == 0)
+    np.testing.assert_allclose(f[order[0]], [26 / 28, 2 / 28, 0, 0], rtol=1e-15)  # cell (1,2,1): index 25

@@ -116,3 +116,51 @@ def test_gpu_device_entry_point():
     assert np.array_equal(d_start.cpu().numpy(), want["cell_start"])
     assert np.array_equal(d_end.cpu().numpy(), want["cell_end"])
     assert np.array_equal(d_sorted.cpu().numpy(), want["sorted_records"])
+
+
+@pytest.mark.gpu
+def test_gpu_device_entry_point_follows_the_callers_stream():
+    """The records are written by kernels still queued on the caller's stream (behind
+    a long chain of matrix products) when the sort is called, with no synchronisation
+    in between: the sort runs after them because it is queued on that stream. The
+    cell arrays are longer than the grid (cell_count): the tail is set to -1."""
+    import torch
+    from aquarius_amd.spatial_hash import SpatialHash
+    rng = np.random.default_rng(4)
+    n, stride, sizes = 300_000, 4, (40, 30, 20)
+    rec = rng.uniform(0, 10, (n, stride)).astype(np.float32)
+    want = R.spatial_hash(rec, sizes, 0.25)
+    cells = sizes[0] * sizes[1] * sizes[2]
+    extra = 777
+    side = torch.cuda.Stream()  # a non-default stream of the caller's (default priority)
+    with torch.cuda.stream(side):
+        src = torch.from_numpy(rec).cuda()
+        d_rec = torch.full_like(src, -1.0)
+        d_start = torch.full((cells + extra,), 5, dtype=torch.int32, device="cuda")
+        d_end = torch.full((cells + extra,), 5, dtype=torch.int32, device="cuda")
+        d_sorted = torch.empty_like(src)
+        torch.cuda.synchronize()
+        a = torch.randn(2048, 2048, device="cuda") / 64
+        for _ in range(40):  # keep the stream busy for a few milliseconds
+            a = torch.tanh(a @ a)
+        d_rec.copy_(src + 0 * a[0, 0])  # written only after the products
+        sh = SpatialHash(sizes, 0.25)
+        sh.sort_device(d_rec.data_ptr(), n, stride, 0, 0, d_start.data_ptr(), d_end.data_ptr(),
+                       d_sorted.data_ptr(), stream=side.cuda_stream, cell_count=cells + extra)
+        got_start = d_start.cpu().numpy()
+        got_end = d_end.cpu().numpy()
+        got_sorted = d_sorted.cpu().numpy()
+        sh.close()
+    assert np.array_equal(got_start[:cells], want["cell_start"])
+    assert np.array_equal(got_end[:cells], want["cell_end"])
+    assert np.all(got_start[cells:] == -1) and np.all(got_end[cells:] == -1)
+    assert np.array_equal(got_sorted, want["sorted_records"])
+
+
+@pytest.mark.gpu
+def test_cell_count_below_the_grid_is_rejected():
+    from aquarius_amd.spatial_hash import SpatialHash
+    sh = SpatialHash((4, 4, 4), 1.0)
+    with pytest.raises(Exception, match="cell_count"):
+        sh.sort(np.zeros((1, 3), np.float32), cell_count=10)
+    sh.close()

@@ -1,8 +1,11 @@
 """The warm-start U-Net: its CPU oracle pinned by finite differences and closed
 forms (CPU), and the GPU forward/backward/Adam kernels checked against the oracle
 (GPU). The GPU keeps bf16 activations and weights with FP32 accumulation; the
-oracle rounds to bf16 at the same points, so the bar is a relative L2 error of
-1e-2 on outputs and 3e-2 on gradients (bf16 has an 8-bit significand)."""
+oracle rounds to bf16 at the same points; what is left is FP32 accumulation order
+and the odd bf16 rounding tie it decides. Observed relative L2 errors are about 2e-4
+on outputs and at most 1.6e-3 on any layer's gradients, so the bars are 1e-3 and
+5e-3: a few times the observed error, well below bf16's 2^-8 = 3.9e-3 relative step
+on outputs. Adam is checked against torch.optim.Adam."""
 import numpy as np
 import pytest
 
@@ -126,7 +129,7 @@ def test_forward_matches_oracle():
     want = U.forward(net.get_params(), x, bf=True)
     err = rel_l2(y, want)
     print(f"forward rel_l2 vs bf16 oracle {err:.2e}; vs fp64 oracle {rel_l2(y, U.forward(net.get_params(), x)):.2e}")
-    assert err < 1e-2
+    assert err < 1e-3
 
 
 @pytest.mark.gpu
@@ -137,14 +140,14 @@ def test_gradients_match_oracle():
     loss, g = net.gradients(x, t, m)
     p = net.get_params()
     want_loss, want = U.loss_and_grads(p, x, t, m, bf=True)
-    assert abs(loss - want_loss) <= 1e-2 * want_loss
+    assert abs(loss - want_loss) <= 2e-3 * want_loss
     blocks, hw, hb, total = U.layout()
     for l, ((w, b), (ci, co)) in enumerate(zip(blocks, U.CONVS)):
         ew = rel_l2(g[w:w + 27 * ci * co], want[w:w + 27 * ci * co])
         eb = rel_l2(g[b:b + co], want[b:b + co])
         print(f"layer {l} ({ci}->{co}): dW rel_l2 {ew:.2e}, db rel_l2 {eb:.2e}")
-        assert ew < 3e-2 and eb < 3e-2, l
-    assert rel_l2(g[hw:], want[hw:]) < 3e-2
+        assert ew < 5e-3 and eb < 5e-3, l
+    assert rel_l2(g[hw:], want[hw:]) < 5e-3
 
 
 @pytest.mark.gpu
@@ -162,6 +165,51 @@ def test_gradient_descent_on_gpu_gradients_reduces_loss():
         net.set_params(p)
         losses.append(loss)
     assert losses[-1] < 0.9 * losses[0], losses
+
+
+@pytest.mark.gpu
+def test_adam_matches_torch_adam():
+    """Three k_adam steps on fixed gradients against torch.optim.Adam (Kingma & Ba,
+    bias-corrected) in FP64 on the CPU from the same parameters: weights, first and
+    second moments. The library folds the bias corrections into the step size
+    (lr*sqrt(1-b2^t)/(1-b1^t)) and adds eps to sqrt(v), where torch adds it to the
+    corrected sqrt(v_hat); with eps far below every sqrt(v) the two forms agree to
+    FP32 rounding, so eps is set to 1e-12 and |g| >= 5e-4 here."""
+    import torch
+    from aquarius_amd.unet import UNet
+    net = UNet(seed=13, eps=1e-12)
+    cfg = net.cfg
+    p0 = net.get_params()
+    rng = np.random.default_rng(21)
+    grads = []
+    for _ in range(3):
+        n = rng.standard_normal(p0.size)
+        grads.append((np.sign(n) * (0.05 + np.abs(n)) * 10.0 ** rng.uniform(-2, 0, p0.size)).astype(np.float32))
+    w = torch.tensor(p0.astype(np.float64), requires_grad=True)
+    opt = torch.optim.Adam([w], lr=float(cfg.lr), betas=(float(cfg.beta1), float(cfg.beta2)),
+                           eps=float(cfg.eps))
+    for g in grads:
+        net.adam_step(g)
+        w.grad = torch.tensor(g.astype(np.float64))
+        opt.step()
+    p = net.get_params()
+    m, v, t = net.adam_state()
+    st = opt.state[w]
+    want = w.detach().numpy()
+    assert t == 3
+    upd = rel_l2(p.astype(np.float64) - p0, want - p0)
+    print(f"Adam update rel_l2 vs torch {upd:.2e}")
+    assert upd < 3e-5
+    # each weight within a few FP32 roundings of its magnitude plus 1e-4 of a step (an
+    # FP32 emulation of the same three steps deviates from torch by at most 2e-5 of a step)
+    ulp = np.spacing(np.maximum(np.abs(p0), np.abs(want)).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(p - want) <= 4 * ulp + 1e-4 * float(cfg.lr))
+    gmax = np.max(np.abs(np.stack(grads)), 0)
+    assert np.all(np.abs(m - st["exp_avg"].numpy()) <= 1e-6 * gmax)
+    np.testing.assert_allclose(v, st["exp_avg_sq"].numpy(), rtol=1e-5, atol=0)
+    # the bf16 weights the kernels use were repacked: the forward pass follows the update
+    x, _, _ = random_problem(B=1, D=8, H=8, W=8, seed=9)
+    assert rel_l2(net.forward(x), U.forward(p, x, bf=True)) < 1e-3
 
 
 @pytest.mark.gpu
