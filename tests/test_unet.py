@@ -207,9 +207,13 @@ def test_adam_matches_torch_adam():
     gmax = np.max(np.abs(np.stack(grads)), 0)
     assert np.all(np.abs(m - st["exp_avg"].numpy()) <= 1e-6 * gmax)
     np.testing.assert_allclose(v, st["exp_avg_sq"].numpy(), rtol=1e-5, atol=0)
-    # the bf16 weights the kernels use were repacked: the forward pass follows the update
+    # the bf16 weights the kernels use were repacked: the forward pass follows the
+    # updated weights (within a few bf16 rounding flips), not the initial ones
     x, _, _ = random_problem(B=1, D=8, H=8, W=8, seed=9)
-    assert rel_l2(net.forward(x), U.forward(p, x, bf=True)) < 1e-3
+    y = net.forward(x)
+    e_new, e_old = rel_l2(y, U.forward(p, x, bf=True)), rel_l2(y, U.forward(p0, x, bf=True))
+    print(f"forward after the steps: rel_l2 {e_new:.2e} vs updated weights, {e_old:.2e} vs initial ones")
+    assert e_new < 5e-3 and e_old > 3 * e_new
 
 
 @pytest.mark.gpu
