@@ -127,6 +127,7 @@ class SolverStats(C.Structure):
         ("last_guess_residual", C.c_double),
         ("last_fallback", C.c_int32),
         ("nn_corrections", C.c_int64),
+        ("nn_rejected", C.c_int64),
     ]
 
 
@@ -184,6 +185,9 @@ PROTOTYPES = {
     "aq_unet_eval": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "aq_flip_nn_guess": (C.c_int, [_P, _P]),
     "aq_flip_nn_correct": (C.c_int, [_P, _P]),
+    "aq_flip_solve_pressure_nn": (C.c_int, [_P, _P, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "aq_flip_set_solver": (C.c_int, [_P, C.POINTER(Solver), _P]),
     "aq_flip_solver_stats_get": (C.c_int, [_P, C.POINTER(SolverStats), C.c_int32]),
     "aq_unet_mfma_probe": (C.c_int, [_FP, _FP, _FP]),

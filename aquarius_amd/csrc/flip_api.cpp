@@ -100,6 +100,7 @@ struct aq_flip {
   aq_flip_solver solver{AQ_SOLVE_FIXED, 1e-4, 100000, 50, 20, 0};
   aq_unet* net = nullptr;
   aq_flip_solver_stats sstats{};
+  float* p_keep = nullptr;  // the iterate before a U-Net correction (allocated on first use)
 
   bool timing = false;
   double stage_ms[AQ_STAGE_COUNT]{};
@@ -677,6 +678,58 @@ aq_status sweep_to_tol(aq_flip* h, int budget, bool detect_stall, int* sweeps, d
   return AQ_OK;
 }
 
+// Refinement of a U-Net guess already in the pressure buffer: Jacobi to the solver's
+// tol within `budget` sweeps, the network correcting the iterate from its residual
+// every nn_refresh sweeps (0: never). A correction is judged after the sweeps that
+// follow it: the network removes smooth error that Jacobi is slow on while adding
+// high-frequency residual that Jacobi removes quickly, so the residual right after a
+// correction says little. If the residual after those sweeps is not below the one
+// before the correction, the iterate from before the correction is restored bit for
+// bit and Jacobi continues alone (near FP32 Jacobi's round-off floor the network's
+// output is mostly noise).
+aq_status nn_refine(aq_flip* h, aq_unet* net, int budget, int nn_refresh, int* total, double* res,
+                    bool* stalled, bool* fallback, int64_t* corrections, int64_t* rejected) {
+  const size_t bytes = (size_t)h->G.ncell * sizeof(float);
+  int left = budget, sweeps = 0;
+  bool correcting = nn_refresh > 0;
+  bool judging = false;  // the last chunk followed a correction
+  double before = 0;     // residual of the iterate saved in p_keep
+  int pb_keep = 0;
+  bool clean_keep = false;
+  *total = 0;
+  for (;;) {
+    const int chunk = correcting ? std::min(nn_refresh, left) : left;
+    AQ_TRY(sweep_to_tol(h, chunk, true, &sweeps, res, stalled));
+    *total += sweeps;
+    left -= sweeps;
+    if (judging) {
+      judging = false;
+      if (!(*res <= h->solver.tol) && !(*res < before)) {
+        // restore the iterate from before the correction
+        HIP_TRY(hipMemcpyAsync(h->gr.p[pb_keep], h->p_keep, bytes, hipMemcpyDeviceToDevice, h->stream));
+        h->pb = pb_keep;
+        h->air_clean[pb_keep] = clean_keep;
+        *res = before;
+        *stalled = false;
+        *rejected += 1;
+        correcting = false;
+        if (left > 0) continue;
+      }
+    }
+    if (!std::isfinite(*res)) *fallback = true;
+    if (*fallback || *res <= h->solver.tol || left <= 0 || *stalled || !correcting) break;
+    if (!h->p_keep) AQ_TRY(h->allocT(&h->p_keep, (size_t)h->G.ncell));
+    pb_keep = h->pb;
+    clean_keep = h->air_clean[h->pb];
+    HIP_TRY(hipMemcpyAsync(h->p_keep, h->gr.p[h->pb], bytes, hipMemcpyDeviceToDevice, h->stream));
+    before = *res;
+    AQ_TRY(aq_flip_nn_correct(h, net));
+    *corrections += 1;
+    judging = true;
+  }
+  return AQ_OK;
+}
+
 // The pressure solve of one step, per the selected mode.
 aq_status pressure_stage(aq_flip* h) {
   aq_flip_solver_stats& st = h->sstats;
@@ -703,17 +756,9 @@ aq_status pressure_stage(aq_flip* h) {
     if (!std::isfinite(guess)) fallback = true;
     // Refine with Jacobi; every nn_refresh sweeps (if set) the network corrects the
     // iterate from its residual, then Jacobi continues.
-    int left = h->solver.max_iters;
-    while (!fallback) {
-      const int budget = h->solver.nn_refresh > 0 ? std::min(h->solver.nn_refresh, left) : left;
-      AQ_TRY(sweep_to_tol(h, budget, true, &sweeps, &res, &stalled));
-      total += sweeps;
-      left -= sweeps;
-      if (!std::isfinite(res)) fallback = true;
-      if (fallback || res <= h->solver.tol || left <= 0 || stalled || h->solver.nn_refresh <= 0) break;
-      AQ_TRY(aq_flip_nn_correct(h, h->net));
-      st.nn_corrections += 1;
-    }
+    if (!fallback)
+      AQ_TRY(nn_refine(h, h->net, h->solver.max_iters, h->solver.nn_refresh, &total, &res, &stalled,
+                       &fallback, &st.nn_corrections, &st.nn_rejected));
   }
   if (h->solver.mode == AQ_SOLVE_MGPCG) {
     const bool from_current = h->cfg.pressure_init == AQ_PRESSURE_PREVIOUS;
@@ -1062,6 +1107,32 @@ aq_status aq_flip_set_solver(aq_flip* h, const aq_flip_solver* s, aq_unet* net) 
     return fail(AQ_EINVAL, "the U-Net needs grid extents that are multiples of 4");
   h->solver = *s;
   h->net = net;
+  return AQ_OK;
+}
+
+aq_status aq_flip_solve_pressure_nn(aq_flip* h, aq_unet* net, double tol, int32_t max_iters,
+                                    int32_t check_every, int32_t nn_refresh, int32_t* iters_out,
+                                    double* rel_out, int32_t* corrections_out, int32_t* rejected_out) {
+  if (!h || !net || max_iters < 0 || check_every < 1 || nn_refresh < 0) return fail(AQ_EINVAL, "bad argument");
+  const aq_flip_solver keep = h->solver;
+  h->solver.tol = tol;
+  h->solver.check_every = check_every;
+  int total = 0;
+  double res = 0;
+  bool stalled = false, fallback = false;
+  int64_t corr = 0, rej = 0;
+  aq_status st = aq_flip_nn_guess(h, net);
+  if (st == AQ_OK) {
+    h->air_clean[h->pb] = true;  // the guess is 0 on AIR cells
+    st = nn_refine(h, net, max_iters, nn_refresh, &total, &res, &stalled, &fallback, &corr, &rej);
+  }
+  h->solver = keep;
+  if (st != AQ_OK) return st;
+  if (iters_out) *iters_out = total;
+  if (rel_out) *rel_out = res;
+  if (corrections_out) *corrections_out = (int32_t)corr;
+  if (rejected_out) *rejected_out = (int32_t)rej;
+  if (!(res <= tol)) return fail(AQ_ESTALL, fallback ? "U-Net refinement is not finite" : "pressure solve did not reach tol");
   return AQ_OK;
 }
 

@@ -133,6 +133,20 @@ def nn_correct(flip, net: UNet) -> None:
     check(load().aq_flip_nn_correct(flip.h, net.h))
 
 
+def solve_pressure_nn(flip, net: UNet, tol: float, max_iters: int, check_every: int = 10,
+                      nn_refresh: int = 0):
+    """The U-Net guess refined by Jacobi with a correction every nn_refresh sweeps
+    (aq_flip_solve_pressure_nn). Returns (sweeps, residual, converged, corrections,
+    corrections undone)."""
+    from . import _abi
+    it, res, nc, nr = C.c_int32(), C.c_double(), C.c_int32(), C.c_int32()
+    st = load().aq_flip_solve_pressure_nn(flip.h, net.h, tol, max_iters, check_every, nn_refresh,
+                                          C.byref(it), C.byref(res), C.byref(nc), C.byref(nr))
+    if st not in (_abi.AQ_OK, _abi.AQ_ESTALL):
+        check(st)
+    return it.value, res.value, st == _abi.AQ_OK, nc.value, nr.value
+
+
 def attach_solver(flip, mode: str, tol: float = 1e-4, max_iters: int = 100000,
                   check_every: int = 50, stall_checks: int = 20, net: UNet | None = None,
                   nn_refresh: int = 0) -> None:

@@ -92,19 +92,12 @@ def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int =
 
 def solve_with_corrections(h: FlipHandle, net: UNet, tol: float, cap: int, refresh: int,
                            check_every: int = 10):
-    """From the U-Net guess: Jacobi in chunks of `refresh` sweeps, the network
-    correcting the iterate from its residual between chunks. Returns (sweeps,
-    residual, converged, corrections)."""
-    from .unet import nn_correct, nn_guess
-    nn_guess(h, net)
-    total, corrections = 0, 0
-    while True:
-        it, rel, ok = h.solve_pressure(tol, min(refresh, cap - total), check_every)
-        total += it
-        if ok or total >= cap:
-            return total, rel, ok, corrections
-        nn_correct(h, net)
-        corrections += 1
+    """From the U-Net guess: Jacobi with a network correction of the iterate every
+    `refresh` sweeps -- the library's AQ_SOLVE_TOL_NN controller
+    (aq_flip_solve_pressure_nn). Returns (sweeps, residual, converged, corrections)."""
+    from .unet import solve_pressure_nn
+    it, rel, ok, nc, _ = solve_pressure_nn(h, net, tol, cap, check_every, refresh)
+    return it, rel, ok, nc
 
 
 def compare_on(h: FlipHandle, net: UNet, tol: float = 1e-4, cap: int = 200000, check_every: int = 10,

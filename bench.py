@@ -14,15 +14,29 @@ Prints one JSON line with the contract fields plus:
                 the library's own stream over the timed region)
   kernels       the same figure for every stage of the step
   jacobi_to_tol plain-Jacobi sweeps from zero to a 1e-4 relative FP64 residual on
-                the last step's system, and (with the trained U-Net weights present)
-                the sweeps from the U-Net guess to the same residual
+                one fixed system -- the pressure system of step 110 of a fresh
+                handle of the workload, whatever --steps is -- and (with the trained
+                U-Net weights present) the sweeps from the U-Net guess alone and from
+                the guess with a correction every 200 sweeps, with the corrections'
+                cost in sweep-equivalents; MG-PCG on the same system
   tolerance_mode steps/s when every step solves to 1e-4: plain Jacobi from zero
-                against the U-Net warm start
+                against the U-Net guess alone, the guess with corrections, and
+                MG-PCG, over steps 2-11 (ball in free fall) and again over steps
+                162-171 (after the ball has met the pool)
+  fp64_error    one step of the workload on the GPU (FP32) against the FP64 CPU
+                oracle from the same seeded particles: relative L2 of pressure, face
+                velocities and particle positions and velocities
   paired_diffusion  diffusion sweeps paired with pressure sweeps per step, how many of
                 them computed their diffusion part in the profiled pass (a sweep after
                 one that changed no bit skips the identical work), and the steps/s of
                 the timed region with that skipping switched off
-  cpu_baseline  the CPU oracle timed on a bounded sample of the same workload
+  cpu_baseline  the CPU oracle timed on a bounded sample of the same workload: after
+                2 warm-up steps, the median of 5 steps on all the threads the box
+                gives this process and the median of 5 on one thread, with per-stage
+                milliseconds, the CPU model and nproc
+
+--timed-only runs the warm-up, the timed region and the profiled pass and nothing
+else (the form a rocprofv3 kernel trace of the timed configuration is taken with).
 """
 from __future__ import annotations
 
@@ -174,20 +188,29 @@ def load_warmstart_net():
     return net
 
 
-def tolerance_mode_steps(cfg, steps: int, cap: int):
-    """Steps/s with every step's pressure solved to a 1e-4 FP64 relative residual
-    (or until FP32 Jacobi stalls at its round-off floor): plain Jacobi from zero
-    against the U-Net warm start with a network correction every NN_REFRESH_TIMED sweeps
-    (zero-guess fallback), on two fresh handles of the same workload."""
+def tolerance_mode_steps(cfg, steps: int, cap: int, advance: int = 0):
+    """Steps/s with every step's pressure solved to a 1e-4 FP64 relative residual (or
+    until FP32 Jacobi stalls at its round-off floor), one fresh handle per solver:
+    plain Jacobi from zero; the U-Net guess alone; the guess with a network correction
+    every NN_REFRESH_TIMED sweeps (a correction that does not lower the residual is
+    undone and Jacobi continues alone); MG-preconditioned CG (FP64) to the same
+    tolerance. `advance` fixed-sweep steps run first (graph replay, untimed)."""
     from aquarius_amd.fluid import FlipHandle
     from aquarius_amd.unet import attach_solver, solver_stats
     net = load_warmstart_net()
-    modes = [("plain", "tol_zero", None)] + ([("nn", "tol_nn", net)] if net is not None else [])
-    out = {"tol": 1e-4, "steps": steps, "check_every": 50, "nn_refresh": NN_REFRESH_TIMED}
-    for label, mode, nnet in modes:
+    modes = [("plain", "tol_zero", None, 0), ("mgpcg", "mgpcg", None, 0)]
+    if net is not None:
+        modes += [("nn_guess_only", "tol_nn", net, 0), ("nn", "tol_nn", net, NN_REFRESH_TIMED)]
+    out = {"tol": 1e-4, "steps": steps, "first_step": advance + 2, "check_every": 50,
+           "nn_refresh": NN_REFRESH_TIMED}
+    for label, mode, nnet, refresh in modes:
         h = FlipHandle(cfg)
-        attach_solver(h, mode, tol=1e-4, max_iters=cap, check_every=50, net=nnet,
-                      nn_refresh=NN_REFRESH_TIMED if nnet is not None else 0)
+        if advance:
+            h.steps(advance)
+        if mode == "mgpcg":
+            attach_solver(h, mode, tol=1e-4, max_iters=500, check_every=1)
+        else:
+            attach_solver(h, mode, tol=1e-4, max_iters=cap, check_every=50, net=nnet, nn_refresh=refresh)
         h.steps(1)  # warm-up (first solve, lazy allocations)
         solver_stats(h, reset=True)
         t0 = time.perf_counter()
@@ -195,28 +218,170 @@ def tolerance_mode_steps(cfg, steps: int, cap: int):
         secs = time.perf_counter() - t0
         st = solver_stats(h)
         h.close()
-        out[label] = {"steps_per_s": round(steps / secs, 3), "mean_sweeps": st["sweeps"] / max(st["solves"], 1),
-                      "fallbacks": st["fallbacks"], "capped": st["capped"],
-                      "nn_corrections": st["nn_corrections"], "last_residual": st["last_residual"]}
-    if "nn" in out:
-        out["speedup"] = round(out["nn"]["steps_per_s"] / out["plain"]["steps_per_s"], 3)
-        out["sweep_ratio"] = round(out["plain"]["mean_sweeps"] / max(out["nn"]["mean_sweeps"], 1), 3)
+        row = {"steps_per_s": round(steps / secs, 3), "capped": st["capped"],
+               "last_residual": st["last_residual"]}
+        if mode == "mgpcg":
+            row["mean_cg_iters"] = st["sweeps"] / max(st["solves"], 1)
+        else:
+            row.update({"mean_sweeps": st["sweeps"] / max(st["solves"], 1), "fallbacks": st["fallbacks"]})
+        if nnet is not None:
+            row.update({"nn_corrections_per_step": st["nn_corrections"] / max(st["solves"], 1),
+                        "nn_rejected": st["nn_rejected"]})
+        out[label] = row
+    for label in ("nn_guess_only", "nn", "mgpcg"):
+        if label in out:
+            out[label]["speedup_vs_plain"] = round(out[label]["steps_per_s"] / out["plain"]["steps_per_s"], 3)
+            if "mean_sweeps" in out[label]:
+                out[label]["sweep_ratio_vs_plain"] = round(
+                    out["plain"]["mean_sweeps"] / max(out[label]["mean_sweeps"], 1), 3)
     return out
 
 
-def cpu_baseline(scene: str, steps: int) -> dict:
+def solve_comparison(cfg, at_step: int, cap: int) -> dict:
+    """Sweeps to a 1e-4 FP64 relative residual on one fixed system: the pressure
+    system after `at_step` steps of a fresh handle (fixed sweeps, graph replay)."""
+    import numpy as np
+
+    from aquarius_amd.fluid import FlipHandle
+    h = FlipHandle(cfg)
+    h.steps(at_step)
+    zero = np.zeros(h.field_shape("pressure"), np.float32)
+    h.set_field("pressure", zero)
+    ts = time.perf_counter()
+    it, rel, ok = h.solve_pressure(1e-4, cap, check_every=100)
+    solve = {"system": f"pressure system of step {at_step} of a fresh handle", "tol": 1e-4,
+             "plain_iters": it, "plain_residual": rel, "plain_converged": ok,
+             "plain_seconds": round(time.perf_counter() - ts, 3), "cap": cap, "nn_warm_iters": None}
+    net = load_warmstart_net()
+    if net is not None:
+        from aquarius_amd import warmstart
+        from aquarius_amd.unet import nn_correct, nn_guess
+        # guess only: one inference, then plain Jacobi
+        ts = time.perf_counter()
+        nn_guess(h, net)
+        g = h.residual()
+        it2, rel2, ok2 = h.solve_pressure(1e-4, cap, check_every=10)
+        solve.update({"nn_warm_iters": it2, "nn_guess_residual": g, "nn_residual": rel2,
+                      "nn_converged": ok2, "nn_seconds": round(time.perf_counter() - ts, 3),
+                      "iter_ratio": round(it / max(it2, 1), 3)})
+        # guess plus a network correction of the iterate every NN_REFRESH sweeps
+        ts = time.perf_counter()
+        it3, rel3, ok3, nc = warmstart.solve_with_corrections(h, net, 1e-4, cap, NN_REFRESH)
+        solve.update({"nn_corrected_iters": it3, "nn_corrections": nc, "nn_corrected_residual": rel3,
+                      "nn_corrected_converged": ok3,
+                      "nn_corrected_seconds": round(time.perf_counter() - ts, 3),
+                      "corrected_iter_ratio": round(it / max(it3, 1), 3), "nn_refresh": NN_REFRESH})
+        # what a correction costs, in sweeps of this system: 10 corrections and 1000
+        # sweeps timed on the host around synchronised calls
+        h.set_field("pressure", zero)
+        h.solve_pressure(0.0, 100, check_every=100)
+        ts = time.perf_counter()
+        for _ in range(10):
+            nn_correct(h, net)
+        t_corr = (time.perf_counter() - ts) / 10
+        ts = time.perf_counter()
+        h.solve_pressure(0.0, 1000, check_every=1000)
+        t_sweep = (time.perf_counter() - ts) / 1000
+        sweq = t_corr / t_sweep
+        solve.update({"correction_ms": round(1e3 * t_corr, 4), "sweep_us": round(1e6 * t_sweep, 3),
+                      "correction_sweep_equivalents": round(sweq, 1),
+                      "guess_only_work_ratio": round(it / (it2 + sweq), 3),
+                      "corrected_work_ratio": round(it / (it3 + (nc + 1) * sweq), 3)})
+    # the accuracy reference: MG-preconditioned CG (FP64) from zero on the same system
+    ts = time.perf_counter()
+    itm, relm, okm = h.solve_pressure_mgpcg(1e-9, max_iters=200)
+    solve.update({"mgpcg_iters": itm, "mgpcg_residual_of_fp32_field": relm, "mgpcg_converged": okm,
+                  "mgpcg_seconds": round(time.perf_counter() - ts, 4)})
+    h.close()
+    return solve
+
+
+def fp64_error(cfg) -> dict:
+    """One step on the GPU (FP32) against one step of the FP64 CPU oracle from the
+    same seeded particles: relative L2 differences. Particles are matched through the
+    sort permutations (both runs start from the same particle order)."""
+    import numpy as np
+
+    from aquarius_amd.fluid import FlipHandle
+    from oracle import binding as ob
+    ob.set_threads(cpu_threads())
+    g = FlipHandle(cfg)
+    o = ob.Oracle(cfg, fp64=True)
+    g.step()
+    o.step()
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    out = {"reference": "CPU oracle in FP64 (same kernels' logic), one step from the same particles"}
+    for f in ("pressure", "new_velocity", "velocity"):
+        out[f] = rel(g.get_field(f), o.get_field(f))
+    gp, gv, _ = g.get_particles()
+    op, ov, _ = o.get_particles()
+    _, _, _, pg = g.get_cells()
+    _, _, _, po = o.get_cells()
+    ig, io = np.empty_like(pg), np.empty_like(po)
+    ig[pg] = np.arange(pg.size)
+    io[po] = np.arange(po.size)
+    out["particle_position"] = rel(gp[ig], op[io])
+    out["particle_velocity"] = rel(gv[ig], ov[io])
+    out["sort_slots_differing"] = int((pg != po).sum())
+    g.close()
+    o.close()
+    return {k: (round(v, 12) if isinstance(v, float) else v) for k, v in out.items()}
+
+
+def cpu_threads() -> int:
+    """Threads the box gives this process (OMP_NUM_THREADS where set, else up to 16)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else min(16, os.cpu_count() or 1)
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene: str, steps: int, warmup: int = 2) -> dict:
+    """The CPU oracle (OpenMP) on the same workload: `warmup` steps, then the median of
+    `steps` steps on all the process's threads and the median of `steps` on one
+    thread, each step timed stage by stage with std::chrono::steady_clock."""
+    import statistics
+
     from aquarius_amd import configs
     from oracle import binding as ob
-    threads = min(16, os.cpu_count() or 1)
-    ob.set_threads(threads)
+    threads = cpu_threads()
     cfg = configs.SCENES[scene]()
+    ob.set_threads(threads)
     o = ob.Oracle(cfg)
-    secs = o.time_steps(steps)
+    for _ in range(warmup):
+        o.step()
+
+    def run(nthreads):
+        ob.set_threads(nthreads)
+        totals, stages = [], []
+        for _ in range(steps):
+            t, st = o.time_step_stages()
+            totals.append(t)
+            stages.append(st)
+        med = statistics.median(totals)
+        per_stage = {k: round(1e3 * statistics.median(s[k] for s in stages), 2) for k in stages[0]}
+        return med, per_stage
+    med_all, stage_all = run(threads)
+    med_one, stage_one = run(1)
     o.close()
-    return {"value": round(steps / secs, 4), "unit": "steps/s", "cores": threads,
-            "kind": "port",
-            "sample": (f"{steps} steps of {scene} after init, CPU oracle (C++ restatement of the "
-                       f"reference kernels, FP32) with {threads} OpenMP threads; {secs:.2f} s")}
+    return {"value": round(1.0 / med_all, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": (f"{scene}: {warmup} warm-up steps, then the median of {steps} steps with {threads} "
+                       f"OpenMP threads and of {steps} more on 1 thread; CPU oracle (C++ restatement of "
+                       f"the reference kernels, FP32)"),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "stage_ms": stage_all,
+            "one_thread": {"value": round(1.0 / med_one, 4), "unit": "steps/s", "cores": 1, "stage_ms": stage_one}}
 
 
 def main() -> int:
@@ -225,12 +390,19 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scene", default="multiphase_128")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed CPU oracle steps per thread count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-cap", type=int, default=60000)
     ap.add_argument("--tol-steps", type=int, default=10,
                     help="steps per solver mode in the tolerance-mode comparison (0: skip)")
+    ap.add_argument("--solve-step", type=int, default=110,
+                    help="the step whose pressure system jacobi_to_tol solves (fresh handle)")
+    ap.add_argument("--contact-step", type=int, default=160,
+                    help="fixed-sweep steps before the post-contact tolerance-mode run")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--timed-only", action="store_true",
+                    help="warm-up, timed region and profiled pass only (for kernel traces)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -262,7 +434,7 @@ def main() -> int:
     # The same timed region with paired diffusion sweeps always computed (no skipping of
     # sweeps that would recompute unchanged values), on a second handle of the same scene.
     skip_off = None
-    if sweeps_paired_possible(cfg, K):
+    if sweeps_paired_possible(cfg, K) and not args.timed_only:
         os.environ["AQFLIP_NO_FIXED_POINT_SKIP"] = "1"
         try:
             h2 = FlipHandle(cfg)
@@ -310,44 +482,16 @@ def main() -> int:
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": dk["frac_of_hbm_peak"],
                 "traffic": measured_traffic(args.scene, dominant)}
 
-    solve = None
-    if not args.no_solve:
-        # plain Jacobi from zero on the last step's system, to a 1e-4 FP64 residual
-        h.set_field("pressure", np.zeros(h.field_shape("pressure"), np.float32))
-        ts = time.perf_counter()
-        it, rel, ok = h.solve_pressure(1e-4, args.solve_cap, check_every=100)
-        solve = {"tol": 1e-4, "plain_iters": it, "plain_residual": rel, "plain_converged": ok,
-                 "plain_seconds": round(time.perf_counter() - ts, 3), "cap": args.solve_cap,
-                 "nn_warm_iters": None}
-        net = load_warmstart_net()
-        if net is not None:
-            from aquarius_amd import warmstart
-            from aquarius_amd.unet import nn_guess
-            # guess only: one inference, then plain Jacobi
-            ts = time.perf_counter()
-            nn_guess(h, net)
-            g = h.residual()
-            it2, rel2, ok2 = h.solve_pressure(1e-4, args.solve_cap, check_every=10)
-            solve.update({"nn_warm_iters": it2, "nn_guess_residual": g, "nn_residual": rel2,
-                          "nn_converged": ok2, "nn_seconds": round(time.perf_counter() - ts, 3),
-                          "iter_ratio": round(it / max(it2, 1), 3)})
-            # guess plus a network correction of the iterate every NN_REFRESH sweeps
-            ts = time.perf_counter()
-            it3, rel3, ok3, nc = warmstart.solve_with_corrections(h, net, 1e-4, args.solve_cap, NN_REFRESH)
-            solve.update({"nn_corrected_iters": it3, "nn_corrections": nc, "nn_corrected_residual": rel3,
-                          "nn_corrected_converged": ok3,
-                          "nn_corrected_seconds": round(time.perf_counter() - ts, 3),
-                          "corrected_iter_ratio": round(it / max(it3, 1), 3), "nn_refresh": NN_REFRESH})
-        # the accuracy reference: MG-preconditioned CG (FP64) from zero on the same system
-        ts = time.perf_counter()
-        itm, relm, okm = h.solve_pressure_mgpcg(1e-9, max_iters=200)
-        solve.update({"mgpcg_iters": itm, "mgpcg_residual_of_fp32_field": relm, "mgpcg_converged": okm,
-                      "mgpcg_seconds": round(time.perf_counter() - ts, 4)})
     h.close()
-
-    tol_mode = None
-    if not args.no_solve and args.tol_steps > 0:
-        tol_mode = tolerance_mode_steps(cfg, args.tol_steps, args.solve_cap)
+    solve = tol_mode = fp64 = None
+    if not args.no_solve and not args.timed_only:
+        solve = solve_comparison(cfg, args.solve_step, args.solve_cap)
+        if args.tol_steps > 0:
+            tol_mode = {"free_fall": tolerance_mode_steps(cfg, args.tol_steps, args.solve_cap),
+                        "after_contact": tolerance_mode_steps(cfg, args.tol_steps, args.solve_cap,
+                                                              advance=args.contact_step)}
+    if not args.no_fp64 and not args.timed_only:
+        fp64 = fp64_error(cfg)
 
     out = {
         "metric": METRIC,
@@ -375,9 +519,10 @@ def main() -> int:
                              "steps_per_s_skip_off": skip_off},
         "jacobi_to_tol": solve,
         "tolerance_mode": tol_mode,
+        "fp64_error": fp64,
         "profiled_pass_ms_per_step": round(eager_ms_per_step, 4),
     }
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.timed_only:
         out["cpu_baseline"] = cpu_baseline(args.scene, args.cpu_steps)
     print(json.dumps(out), flush=True)
     return 0

@@ -96,6 +96,19 @@ aq_status aq_flip_nn_guess(aq_flip* flip, aq_unet* net);
  * relative to the residual equation), then p += d on FLUID cells. */
 aq_status aq_flip_nn_correct(aq_flip* flip, aq_unet* net);
 
+/* The AQ_SOLVE_TOL_NN solve on the handle's current system, outside a step: the U-Net
+ * guess, then Jacobi to tol (FP64 relative residual checked every check_every sweeps,
+ * at most max_iters sweeps, stalls detected as the solver's stall_checks says) with a
+ * U-Net correction every nn_refresh sweeps (0: none). A correction is judged by the
+ * residual after the sweeps that follow it; if that is not below the residual before
+ * the correction, the iterate from before it is restored bit for bit and Jacobi
+ * continues without further corrections. Outputs may be NULL: sweeps, the final
+ * residual, corrections applied and corrections undone. AQ_ESTALL when tol was not
+ * reached. */
+aq_status aq_flip_solve_pressure_nn(aq_flip* flip, aq_unet* net, double tol, int32_t max_iters,
+                                    int32_t check_every, int32_t nn_refresh, int32_t* iters,
+                                    double* residual, int32_t* corrections, int32_t* rejected);
+
 /* FIXED: cfg.pressure_iters Jacobi sweeps (the reference). TOL_ZERO: Jacobi from zero
  * to tol. TOL_NN: Jacobi from the U-Net guess to tol with U-Net corrections.
  * MGPCG: multigrid-preconditioned CG in FP64 to tol (aq_flip_solve_pressure_mgpcg;
@@ -122,6 +135,9 @@ typedef struct aq_flip_solver_stats {
   double last_guess_residual; /* residual of the last initial guess */
   int32_t last_fallback;
   int64_t nn_corrections; /* U-Net residual corrections applied */
+  int64_t nn_rejected;    /* of those, undone: the residual after the sweeps that
+                             followed was not below the one before the correction
+                             (the solve then continues with Jacobi alone) */
 } aq_flip_solver_stats;
 /* Select how aq_flip_step solves for pressure; net is required for
  * AQ_SOLVE_TOL_NN and must outlive its use. */

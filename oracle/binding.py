@@ -55,6 +55,7 @@ def load() -> C.CDLL:
         "orc_residual": (C.c_double, [P]),
         "orc_physical_time": (C.c_double, [P]),
         "orc_time_steps": (C.c_double, [P, C.c_int]),
+        "orc_time_step_stages": (C.c_double, [P, D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -190,3 +191,12 @@ class Oracle:
 
     def time_steps(self, n: int) -> float:
         return self.lib.orc_time_steps(self.h, n)
+
+    STAGE_NAMES = ("g2p", "advect", "transfer_to_grid", "gravity_boundary", "divergence", "pressure",
+                   "project", "extrapolate", "diffusion")
+
+    def time_step_stages(self):
+        """One step, timed: (total seconds, {stage: seconds})."""
+        acc = np.zeros(9)
+        total = self.lib.orc_time_step_stages(self.h, acc.ctypes.data_as(C.POINTER(C.c_double)))
+        return total, dict(zip(self.STAGE_NAMES, acc.tolist()))

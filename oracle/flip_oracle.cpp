@@ -232,6 +232,17 @@ double orc_physical_time(void* p) {
 }
 
 // Wall-clock seconds of `steps` full steps (CPU baseline timing).
+// One step timed stage by stage (steady_clock): adds each stage's seconds to
+// stage_seconds[9] (see Oracle::stepTimed) and returns the step's total.
+double orc_time_step_stages(void* p, double* stage_seconds) {
+  double local[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  auto t0 = std::chrono::steady_clock::now();
+  dispatch<void>((Handle*)p, [&](auto& o) { o.template stepTimed<std::chrono::steady_clock>(local); });
+  auto t1 = std::chrono::steady_clock::now();
+  for (int k = 0; k < 9; ++k) stage_seconds[k] += local[k];
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
 double orc_time_steps(void* p, int steps) {
   auto t0 = std::chrono::steady_clock::now();
   for (int i = 0; i < steps; ++i) orc_step(p);

@@ -30,6 +30,7 @@
 #pragma once
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -766,6 +767,40 @@ struct Oracle {
     project();
     extrapolate();
     diffusion(cfg.diffusion_iters);
+    physicalTime += cfg.dt;
+  }
+
+  // The same step with the wall-clock seconds of each of its stages added to out[9]:
+  // g2p, advect, transfer to grid (hash, sort, P2G), gravity + boundary, count +
+  // divergence, pressure sweeps, projection, extrapolation, diffusion sweeps.
+  template <typename Clock>
+  void stepTimed(double out[9]) {
+    auto t = Clock::now();
+    auto lap = [&](int k) {
+      const auto n = Clock::now();
+      out[k] += std::chrono::duration<double>(n - t).count();
+      t = n;
+    };
+    g2p();
+    lap(0);
+    advect();
+    lap(1);
+    transferToGrid();
+    lap(2);
+    applyGravity();
+    fixBoundary();
+    lap(3);
+    countParticles();
+    computeDivergence();
+    lap(4);
+    solvePressure(cfg.pressure_iters);
+    lap(5);
+    project();
+    lap(6);
+    extrapolate();
+    lap(7);
+    diffusion(cfg.diffusion_iters);
+    lap(8);
     physicalTime += cfg.dt;
   }
 
