@@ -200,6 +200,8 @@ PROTOTYPES = {
     "aq_hash_destroy": (None, [_P]),
     "aq_hash_sort_host": (C.c_int, [_P, _FP, C.c_int64, C.c_int32, _IP, _IP, _IP, _IP, C.c_int64, _FP]),
     "aq_hash_sort_device": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P, C.c_int64, _P, _P]),
+    "aq_hash_sort_device_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_float, _P, C.c_int64,
+                                              C.c_int32, _P, _P, _P, _P, C.c_int64, _P, _P]),
 }
 
 _lib = None

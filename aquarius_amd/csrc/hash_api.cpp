@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -120,6 +121,70 @@ aq_status check_cell_count(const aq_hash* h, long long cell_count) {
   return AQ_OK;
 }
 
+aq_status check_grid(int sx, int sy, int sz, float cell_size) {
+  if (sx < 1 || sy < 1 || sz < 1) return hfail(AQ_EINVAL, "grid sizes must be >= 1");
+  if ((long long)sx * sy * sz >= 0x7fffffffLL) return hfail(AQ_EINVAL, "grid too large");
+  if (!(cell_size > 0) || !std::isfinite(cell_size)) return hfail(AQ_EINVAL, "cell_size must be positive");
+  return AQ_OK;
+}
+
+Geom hash_geom(int sx, int sy, int sz, float cell_size) {
+  Geom G{};
+  G.X = sx, G.Y = sy, G.Z = sz;
+  G.ncells_hash = sx * sy * sz;
+  G.h = cell_size;
+  G.inv_h = 1.0 / (double)cell_size;
+  G.divZ = make_fastdiv((uint32_t)sz);
+  G.divYZ = make_fastdiv((uint32_t)(sy * sz));
+  return G;
+}
+
+// ---- the entry point that allocates nothing (aq_hash_sort_device_noalloc)
+// Its bins live in one file-scope device array: per-cell counts and offsets, the scan
+// partials, and per-record hash, rank and permutation, for at most kStaticCells cells
+// and kStaticRecords records (67 MB on every GPU the library runs on, held until the
+// process ends). In host code `scratch` names a host-side stand-in, so the array's
+// device address is looked up once per device with hipGetSymbolAddress and only that
+// address is used from then on.
+constexpr long long kStaticCells = 1LL << 21;    // 128^3
+constexpr long long kStaticRecords = 1LL << 22;  // 4,194,304
+constexpr long long kStaticParts = (kStaticCells + 4095) / 4096 + 1;
+constexpr long long align64(long long v) { return (v + 63) / 64 * 64; }  // ints, 256 bytes
+constexpr long long kOffCount = 0;
+constexpr long long kOffOffsets = kOffCount + align64(kStaticCells);
+constexpr long long kOffParts = kOffOffsets + align64(kStaticCells + 1);
+constexpr long long kOffHash = kOffParts + align64(kStaticParts);
+constexpr long long kOffRank = kOffHash + align64(kStaticRecords);
+constexpr long long kOffPerm = kOffRank + align64(kStaticRecords);
+constexpr long long kStaticInts = kOffPerm + align64(kStaticRecords);
+
+}  // namespace
+
+static __device__ __align__(256) int scratch[kStaticInts];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up on the first call
+// made with that device current).
+aq_status static_scratch(int** out) {
+  int dev = -1;
+  HHIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return hfail(AQ_EINVAL, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HHIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return hfail(AQ_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return AQ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -128,18 +193,11 @@ aq_status aq_hash_create(int32_t sx, int32_t sy, int32_t sz, float cell_size, in
                          aq_hash** out) {
   if (!out) return hfail(AQ_EINVAL, "null output");
   *out = nullptr;
-  if (sx < 1 || sy < 1 || sz < 1) return hfail(AQ_EINVAL, "grid sizes must be >= 1");
-  if ((long long)sx * sy * sz >= 0x7fffffffLL) return hfail(AQ_EINVAL, "grid too large");
-  if (!(cell_size > 0) || !std::isfinite(cell_size)) return hfail(AQ_EINVAL, "cell_size must be positive");
+  HTRY(check_grid(sx, sy, sz, cell_size));
   if (capacity < 0) return hfail(AQ_EINVAL, "negative capacity");
   aq_hash* h = new aq_hash;
   Geom& G = h->G;
-  G.X = sx, G.Y = sy, G.Z = sz;
-  G.ncells_hash = sx * sy * sz;
-  G.h = cell_size;
-  G.inv_h = 1.0 / (double)cell_size;
-  G.divZ = make_fastdiv((uint32_t)sz);
-  G.divYZ = make_fastdiv((uint32_t)(sy * sz));
+  G = hash_geom(sx, sy, sz, cell_size);
   auto bail = [&](aq_status s) {
     delete h;
     return s;
@@ -204,6 +262,47 @@ aq_status aq_hash_sort_device(aq_hash* h, const float* d_rec, int64_t n, int32_t
     if (d_end) launch_fill_u32(d_end + ncells, 0xffffffffu, cell_count - ncells, s);
     HHIP(hipGetLastError());
   }
+  HHIP(hipStreamSynchronize(s));
+  return AQ_OK;
+}
+
+aq_status aq_hash_sort_device_noalloc(int32_t sx, int32_t sy, int32_t sz, float cell_size,
+                                      const float* d_rec, int64_t n, int32_t stride, int32_t* d_perm,
+                                      int32_t* d_shash, int32_t* d_start, int32_t* d_end,
+                                      int64_t cell_count, float* d_sorted, void* stream) {
+  HTRY(check_grid(sx, sy, sz, cell_size));
+  const Geom G = hash_geom(sx, sy, sz, cell_size);
+  if (G.ncells_hash > kStaticCells)
+    return hfail(AQ_EINVAL, "aq_hash_sort_device_noalloc: size_x*size_y*size_z exceeds 2097152 cells");
+  if (n < 0 || n > kStaticRecords)
+    return hfail(AQ_EINVAL, "aq_hash_sort_device_noalloc: record count must be in [0, 4194304]");
+  if (n > 0 && !d_rec) return hfail(AQ_EINVAL, "null records");
+  if (stride < 3) return hfail(AQ_EINVAL, "stride must be >= 3 floats");
+  if (cell_count != 0 && cell_count < G.ncells_hash)
+    return hfail(AQ_EINVAL, "cell_count must be 0 or at least size_x*size_y*size_z");
+  if (d_sorted && d_sorted == d_rec) return hfail(AQ_EINVAL, "sorted records must not alias the input");
+  int* base = nullptr;
+  HTRY(static_scratch(&base));
+  Bins B{};
+  B.count = base + kOffCount;
+  B.offsets = base + kOffOffsets;
+  B.partials = base + kOffParts;
+  B.hash = base + kOffHash;
+  B.rank = base + kOffRank;
+  B.perm = base + kOffPerm;
+  const hipStream_t s = (hipStream_t)stream;
+  // The same kernels as aq_hash_sort_device; the counts start from a fill kernel.
+  launch_fill_u32(B.count, 0u, G.ncells_hash, s);
+  if (n > 0) launch_hash_records(G, d_rec, n, stride, B, s);
+  launch_scan(G, B, s);
+  if (n > 0) launch_scatter_segsort(G, n, B, s);
+  launch_hash_outputs(G, n, stride, B, d_rec, d_start, d_end, d_shash, d_sorted, s);
+  if (d_perm && n > 0) HHIP(hipMemcpyAsync(d_perm, B.perm, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+  if (cell_count > G.ncells_hash) {
+    if (d_start) launch_fill_u32(d_start + G.ncells_hash, 0xffffffffu, cell_count - G.ncells_hash, s);
+    if (d_end) launch_fill_u32(d_end + G.ncells_hash, 0xffffffffu, cell_count - G.ncells_hash, s);
+  }
+  HHIP(hipGetLastError());
   HHIP(hipStreamSynchronize(s));
   return AQ_OK;
 }

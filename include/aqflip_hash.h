@@ -66,6 +66,22 @@ aq_status aq_hash_sort_device(aq_hash* h, const float* d_records, int64_t n, int
                               int32_t* d_cell_end, int64_t cell_count, float* d_sorted_records,
                               void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): aq_hash_sort_device without a handle, calling no hipMalloc/hipFree.
+ * Its bins (per-cell counts and offsets, per-record hash, rank and permutation) live
+ * in one static device array of 67 MB that every GPU the library runs on holds until
+ * the process ends, so it takes at most 2,097,152 cells (size_x*size_y*size_z) and
+ * 4,194,304 records; larger requests fail with AQ_EINVAL. Inputs, outputs, cell_count
+ * and stream ordering are those of aq_hash_sort_device. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other host threads, other streams)
+ * race and must be serialised by the caller. */
+aq_status aq_hash_sort_device_noalloc(int32_t size_x, int32_t size_y, int32_t size_z, float cell_size,
+                                      const float* d_records, int64_t n, int32_t stride,
+                                      int32_t* d_perm, int32_t* d_sorted_hash, int32_t* d_cell_start,
+                                      int32_t* d_cell_end, int64_t cell_count,
+                                      float* d_sorted_records, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
