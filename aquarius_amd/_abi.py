@@ -177,6 +177,7 @@ PROTOTYPES = {
     "aq_unet_gradients_host": (C.c_int, [_P, _FP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          C.POINTER(C.c_double), _FP]),
     "aq_unet_adam_step_host": (C.c_int, [_P, _FP]),
+    "aq_unet_set_lr": (C.c_int, [_P, C.c_float]),
     "aq_unet_get_adam_state": (C.c_int, [_P, _FP, _FP, C.POINTER(C.c_int64)]),
     "aq_unet_dataset_reserve": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "aq_unet_dataset_add_flip": (C.c_int, [_P, _P]),

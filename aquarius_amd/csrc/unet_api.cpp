@@ -437,6 +437,12 @@ aq_status aq_unet_adam_step_host(aq_unet* net, const float* grads) {
   return AQ_OK;
 }
 
+aq_status aq_unet_set_lr(aq_unet* net, float lr) {
+  if (!net || !(lr >= 0.f) || !std::isfinite(lr)) return ufail(AQ_EINVAL, "lr must be finite and >= 0");
+  net->cfg.lr = lr;
+  return AQ_OK;
+}
+
 aq_status aq_unet_get_adam_state(aq_unet* net, float* m, float* v, int64_t* t) {
   if (!net) return ufail(AQ_EINVAL, "null argument");
   if (m) UHIP(hipMemcpyAsync(m, net->m, net->nparam * sizeof(float), hipMemcpyDeviceToHost, net->stream));

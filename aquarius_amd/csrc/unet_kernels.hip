@@ -12,6 +12,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "unet_internal.h"
 
 namespace aqu {
@@ -57,6 +60,13 @@ __device__ __forceinline__ long long vox_id(const Vol& v, int b, int d, int h, i
 // in LDS once, rows padded by 16 bytes so the 16 rows a wave reads fall in
 // distinct banks. With flipped, transposed weights the same kernel computes the
 // input gradient. OUT_F32 writes FP32 (optionally accumulating), else bf16.
+// AQUNET_NO_TILE=1 switches the LDS-tiled forward convolution off (measurements and
+// the tiled-against-untiled test); read at every launch.
+static bool conv3_tile_enabled() {
+  const char* e = std::getenv("AQUNET_NO_TILE");
+  return !(e && e[0] == '1');
+}
+
 constexpr int kConvMF = 4;
 constexpr int kConvVox = 4 * kConvMF * 16;  // voxels per 256-thread block
 
@@ -152,9 +162,131 @@ __global__ __launch_bounds__(256) void k_conv3(Vol v, const __bf16* __restrict__
     }
 }
 
+// ------------------------------------------------------------------ conv 3x3x3, input tile in LDS
+// The forward convolution of the layers whose extents divide into tiles: a block owns
+// an output tile of TD x 8 x 16 voxels (d, h, w) of one batch item and NTB 16-channel
+// output tiles (channel group blockIdx.y). The input tile with its one-voxel halo,
+// (TD+2) x 10 x 18 voxels, is staged in LDS CH input channels at a time (zero outside
+// the volume), so each input value is read from memory about 2.5 times per layer
+// instead of once per tap; the A fragments of all 27 taps then come from LDS. Each of
+// the four waves owns MF = 2*TD rows of 16 voxels along w. Voxels sit CH + 8 bf16
+// apart in LDS (CH when CH = 8), which puts the 16-byte reads of 8 consecutive lanes in
+// distinct banks. The k order is (chunk, tap, channel within the chunk): for CIN = CH
+// it is the order of k_conv3.
+template <int CIN, int COUT, int CH, int TD, int NTB, bool RELU>
+__global__ __launch_bounds__(256) void k_conv3_tile(Vol v, const __bf16* __restrict__ X, int xs, int xo,
+                                                    const __bf16* __restrict__ Wk,
+                                                    const float* __restrict__ bias, __bf16* __restrict__ Y,
+                                                    int ys, int yo) {
+  constexpr int TH = 8, TW = 16;
+  constexpr int ID = TD + 2, IH = TH + 2, IW = TW + 2;
+  constexpr int NV = ID * IH * IW;
+  constexpr int VS = CH == 8 ? 8 : CH + 8;  // LDS voxel stride (bf16)
+  constexpr int MF = TD * TH / 4;           // 16-voxel rows per wave
+  constexpr int KC = 27 * CH;               // k per channel chunk
+  constexpr int NSL = (KC + 31) / 32;
+  constexpr int K = 27 * CIN;
+  static_assert(CIN % CH == 0 && CH % 8 == 0, "channel chunks of 8, 16 or 32");
+  __shared__ __align__(16) __bf16 sX[NV * VS];
+  const int ntw = v.W / TW, nth = v.H / TH, ntd = v.D / TD;
+  int t = xcd_block();
+  const int tw = t % ntw;
+  t /= ntw;
+  const int th = t % nth;
+  t /= nth;
+  const int td = t % ntd;
+  const int b = t / ntd;
+  const int d0 = td * TD, h0 = th * TH, w0 = tw * TW;
+  const int n0 = (int)blockIdx.y * NTB * 16;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = lane & 15, q = lane >> 4;
+  f32x4 acc[MF][NTB];
+#pragma unroll
+  for (int f = 0; f < MF; ++f)
+#pragma unroll
+    for (int nt = 0; nt < NTB; ++nt) acc[f][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8 zero8 = {};
+  for (int cc = 0; cc < CIN; cc += CH) {
+    if (cc) __syncthreads();  // every wave is done with the previous chunk
+    for (int e = threadIdx.x; e < NV * (CH / 8); e += 256) {
+      const int vx = e / (CH / 8), j = e - vx * (CH / 8);
+      const int iw = vx % IW, ih = (vx / IW) % IH, id = vx / (IW * IH);
+      const int zd = d0 + id - 1, zh = h0 + ih - 1, zw = w0 + iw - 1;
+      bf16x8 val = zero8;
+      if (zd >= 0 && zd < v.D && zh >= 0 && zh < v.H && zw >= 0 && zw < v.W)
+        val = *(const bf16x8*)(X + vox_id(v, b, zd, zh, zw) * xs + xo + cc + j * 8);
+      *(bf16x8*)&sX[vx * VS + j * 8] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      const int kc = s * 32 + q * 8;
+      const bool kok = kc < KC;
+      const int tap = kok ? kc / CH : 0, c = kc - tap * CH;
+      const int dd = tap / 9, dh = (tap / 3) % 3, dw = tap % 3;
+      bf16x8 bw[NTB];
+#pragma unroll
+      for (int nt = 0; nt < NTB; ++nt)
+        bw[nt] = kok ? *(const bf16x8*)(Wk + (long long)(n0 + nt * 16 + r) * K + tap * CIN + cc + c) : zero8;
+#pragma unroll
+      for (int f = 0; f < MF; ++f) {
+        const int row = wave * MF + f;
+        const int rd = row / TH, rh = row - rd * TH;
+        const bf16x8 a =
+            kok ? *(const bf16x8*)&sX[(((rd + dd) * IH + (rh + dh)) * IW + (r + dw)) * VS + c] : zero8;
+#pragma unroll
+        for (int nt = 0; nt < NTB; ++nt)
+          acc[f][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[nt], acc[f][nt], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < MF; ++f) {
+    const int row = wave * MF + f;
+    const int rd = row / TH, rh = row - rd * TH;
+#pragma unroll
+    for (int nt = 0; nt < NTB; ++nt) {
+      const int n = n0 + nt * 16 + r;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const long long o = vox_id(v, b, d0 + rd, h0 + rh, w0 + q * 4 + i);
+        float val = acc[f][nt][i];
+        if (bias) val += bias[n];
+        if (RELU) val = fmaxf(val, 0.f);
+        Y[o * ys + yo + n] = f2bf(val);
+      }
+    }
+  }
+}
+
+// Launches k_conv3_tile when the extents divide into its tiles (W % 16, H % 8 and,
+// for TD = 2, D % 2), with ReLU. Two depths TD and one or two channel tiles per
+// block, chosen so that the grid still has about 512 or more blocks.
+template <int CIN, int COUT, int CH>
+static bool conv3_tile_fwd(const Vol& v, const __bf16* X, int xs, int xo, const __bf16* Wk, const float* bias,
+                           __bf16* Y, int ys, int yo, hipStream_t s) {
+  if (v.W % 16 || v.H % 8) return false;
+  const long long tiles1 = (long long)v.B * v.D * (v.H / 8) * (v.W / 16);
+  const bool td2 = v.D % 2 == 0 && tiles1 / 2 * (COUT / 16) >= 512;
+  const long long tiles = td2 ? tiles1 / 2 : tiles1;
+  const bool ntb2 = COUT >= 32 && tiles * (COUT / 32) >= 512;
+  const dim3 grid((unsigned)tiles, ntb2 ? COUT / 32 : COUT / 16);
+  if (td2 && ntb2)
+    hipLaunchKernelGGL((k_conv3_tile<CIN, COUT, CH, 2, 2, true>), grid, dim3(256), 0, s, v, X, xs, xo, Wk, bias, Y, ys, yo);
+  else if (td2)
+    hipLaunchKernelGGL((k_conv3_tile<CIN, COUT, CH, 2, 1, true>), grid, dim3(256), 0, s, v, X, xs, xo, Wk, bias, Y, ys, yo);
+  else if (ntb2)
+    hipLaunchKernelGGL((k_conv3_tile<CIN, COUT, CH, 1, 2, true>), grid, dim3(256), 0, s, v, X, xs, xo, Wk, bias, Y, ys, yo);
+  else
+    hipLaunchKernelGGL((k_conv3_tile<CIN, COUT, CH, 1, 1, true>), grid, dim3(256), 0, s, v, X, xs, xo, Wk, bias, Y, ys, yo);
+  return true;
+}
+
 template <int CIN, int COUT>
 static void conv3_fwd_t(const Vol& v, const __bf16* X, int xs, int xo, const __bf16* Wk,
                         const float* bias, __bf16* Y, int ys, int yo, bool relu, hipStream_t s) {
+  constexpr int CH = CIN % 32 == 0 ? 32 : CIN % 16 == 0 ? 16 : 8;
+  if (relu && conv3_tile_enabled() && conv3_tile_fwd<CIN, COUT, CH>(v, X, xs, xo, Wk, bias, Y, ys, yo, s)) return;
   const long long V = (long long)v.B * v.D * v.H * v.W;
   if (V >= 1024 * (long long)kConvVox) {  // enough blocks: 4 fragments per wave, all tiles
     const int nb = (int)((V + kConvVox - 1) / kConvVox);
@@ -359,19 +491,31 @@ void relu_bwd(long long V, int C, const __bf16* Y, int ys, int yo, const float* 
 
 // ------------------------------------------------------------------ pooling / upsampling
 // 2x2x2 average pooling (forward, bf16) and its adjoint (accumulating FP32).
+// One thread per output voxel and 8 consecutive channels; the eight children are
+// summed in the same (d, h, w) order for every channel.
 __global__ void k_avgpool(Vol vo, int C, const __bf16* __restrict__ X, int xs, int xo,
-                          __bf16* __restrict__ Y) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long V = (long long)vo.B * vo.D * vo.H * vo.W;
-  if (e >= V * C) return;
-  const int c = (int)(e % C);
-  const VoxelPos p = vox_of(vo, e / C);
+                          __bf16* __restrict__ Y, int row0) {
+  const int C8 = C >> 3;
+  const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (unsigned)vo.W * C8) return;
+  const int w = (int)(e / C8), j = (int)(e - (unsigned)w * C8);
+  unsigned row = row0 + blockIdx.y;  // (b, d, h) of the output
+  const int h = (int)(row % vo.H);
+  row /= vo.H;
+  const int d = (int)(row % vo.D), b = (int)(row / vo.D);
   const Vol vi{vo.B, vo.D * 2, vo.H * 2, vo.W * 2};
-  float s = 0.f;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    s += bf2f(X[vox_id(vi, p.b, 2 * p.d + (k >> 2), 2 * p.h + ((k >> 1) & 1), 2 * p.w + (k & 1)) * xs + xo + c]);
-  Y[e] = f2bf(s * 0.125f);
+  for (int k = 0; k < 8; ++k) {
+    const bf16x8 x8 =
+        *(const bf16x8*)(X + vox_id(vi, b, 2 * d + (k >> 2), 2 * h + ((k >> 1) & 1), 2 * w + (k & 1)) * xs + xo + 8 * j);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] += bf2f(x8[c]);
+  }
+  bf16x8 o8;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) o8[c] = f2bf(acc[c] * 0.125f);
+  *(bf16x8*)(Y + vox_id(vo, b, d, h, w) * C + 8 * j) = o8;
 }
 __global__ void k_avgpool_bwd(Vol vo, int C, const float* __restrict__ dY, float* __restrict__ dX,
                               int dxs, int dxo) {
@@ -390,16 +534,22 @@ __global__ void k_avgpool_bwd(Vol vo, int C, const float* __restrict__ dY, float
 }
 // Nearest 2x upsampling into a channel slice (forward) and its adjoint (sum of the
 // eight children, FP32).
+// One thread per output voxel and 8 consecutive channels (a 16-byte load and store).
 __global__ void k_upsample(Vol vi, int C, const __bf16* __restrict__ X, __bf16* __restrict__ Y, int ys,
-                           int yo) {
-  const Vol vo{vi.B, vi.D * 2, vi.H * 2, vi.W * 2};
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long V = (long long)vo.B * vo.D * vo.H * vo.W;
-  if (e >= V * C) return;
-  const int c = (int)(e % C);
-  const long long id = e / C;
-  const VoxelPos p = vox_of(vo, id);
-  Y[id * ys + yo + c] = X[vox_id(vi, p.b, p.d >> 1, p.h >> 1, p.w >> 1) * C + c];
+                           int yo, int row0) {
+  const int C8 = C >> 3;
+  const int Wo = vi.W * 2, Ho = vi.H * 2, Do = vi.D * 2;
+  const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nrow = (unsigned)Wo * C8;
+  if (e >= nrow) return;
+  const int w = (int)(e / C8), j = (int)(e - (unsigned)w * C8);
+  unsigned row = row0 + blockIdx.y;  // (b, d, h) of the output
+  const int h = (int)(row % Ho);
+  row /= Ho;
+  const int d = (int)(row % Do), b = (int)(row / Do);
+  const long long o = (((long long)b * Do + d) * Ho + h) * Wo + w;
+  const long long i = vox_id(vi, b, d >> 1, h >> 1, w >> 1);
+  *(bf16x8*)(Y + o * ys + yo + 8 * j) = *(const bf16x8*)(X + i * C + 8 * j);
 }
 __global__ void k_upsample_bwd(Vol vi, int C, const float* __restrict__ dY, int dys, int dyo,
                                float* __restrict__ dX) {
@@ -418,16 +568,22 @@ __global__ void k_upsample_bwd(Vol vi, int C, const float* __restrict__ dY, int 
 
 static inline int nb1(long long n) { return (int)((n + 255) / 256); }
 void avgpool(const Vol& vo, int C, const __bf16* X, int xs, int xo, __bf16* Y, hipStream_t s) {
-  const long long n = (long long)vo.B * vo.D * vo.H * vo.W * C;
-  hipLaunchKernelGGL(k_avgpool, dim3(nb1(n)), dim3(256), 0, s, vo, C, X, xs, xo, Y);
+  const int rows = vo.B * vo.D * vo.H;
+  for (int r0 = 0; r0 < rows; r0 += 65535) {  // grid.y limit
+    const dim3 grid((unsigned)((vo.W * (C / 8) + 255) / 256), (unsigned)std::min(65535, rows - r0));
+    hipLaunchKernelGGL(k_avgpool, grid, dim3(256), 0, s, vo, C, X, xs, xo, Y, r0);
+  }
 }
 void avgpool_bwd(const Vol& vo, int C, const float* dY, float* dX, int dxs, int dxo, hipStream_t s) {
   const long long n = (long long)vo.B * vo.D * vo.H * vo.W * C;
   hipLaunchKernelGGL(k_avgpool_bwd, dim3(nb1(n)), dim3(256), 0, s, vo, C, dY, dX, dxs, dxo);
 }
 void upsample(const Vol& vi, int C, const __bf16* X, __bf16* Y, int ys, int yo, hipStream_t s) {
-  const long long n = (long long)vi.B * vi.D * vi.H * vi.W * 8 * C;
-  hipLaunchKernelGGL(k_upsample, dim3(nb1(n)), dim3(256), 0, s, vi, C, X, Y, ys, yo);
+  const int rows = vi.B * vi.D * 2 * vi.H * 2;
+  for (int r0 = 0; r0 < rows; r0 += 65535) {  // grid.y limit
+    const dim3 grid((unsigned)((vi.W * 2 * (C / 8) + 255) / 256), (unsigned)std::min(65535, rows - r0));
+    hipLaunchKernelGGL(k_upsample, grid, dim3(256), 0, s, vi, C, X, Y, ys, yo, r0);
+  }
 }
 void upsample_bwd(const Vol& vi, int C, const float* dY, int dys, int dyo, float* dX, hipStream_t s) {
   const long long n = (long long)vi.B * vi.D * vi.H * vi.W * C;
@@ -440,10 +596,13 @@ __global__ void k_head(long long V, int C, const __bf16* __restrict__ X, const f
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= V) return;
   float s = b[0];
-  for (int c = 0; c < C; ++c) s += w[c] * bf2f(X[i * C + c]);
+  for (int c0 = 0; c0 < C; c0 += 8) {  // C is a multiple of 8: 16-byte loads
+    const bf16x8 x8 = *(const bf16x8*)(X + i * C + c0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) s += w[c0 + c] * bf2f(x8[c]);
+  }
   y[i] = s;
 }
-// dX[v, c] = dy[v] * w[c]; per-block partials of dw[c] = sum dy*X and db = sum dy.
 __global__ void k_head_bwd(long long V, int C, const __bf16* __restrict__ X, const float* __restrict__ w,
                            const float* __restrict__ dy, float* __restrict__ dX, float* __restrict__ part,
                            int vpb) {
@@ -618,10 +777,11 @@ void mfma_probe(const float* A, const float* B, float* D, hipStream_t s) {
 namespace aqu {
 
 __device__ __forceinline__ void cell3(const aqf::Geom& G, long long n, int& x, int& y, int& z) {
-  z = (int)(n % G.Z);
-  const long long t = n / G.Z;
-  y = (int)(t % G.Y);
-  x = (int)(t / G.Y);
+  const uint32_t u = (uint32_t)n;  // n < X*Y*Z < 2^31
+  x = (int)aqf::fast_div(u, G.divYZ);
+  const uint32_t r = u - (uint32_t)x * (uint32_t)(G.Y * G.Z);
+  y = (int)aqf::fast_div(r, G.divZ);
+  z = (int)(r - (uint32_t)y * (uint32_t)G.Z);
 }
 
 // sigma = RMS of the right-hand side b over FLUID cells: sig[0] = sigma, sig[1] = 1/sigma
@@ -866,37 +1026,61 @@ __global__ void k_nn_scatter(aqf::Geom G, const uint8_t* __restrict__ content, c
   p[gi] = content[gi] == aqf::FLUID ? pf[n] : 0.f;
 }
 // <p, b> and <p, A p> over FLUID cells, A as in the FP64 residual.
+// Per block: sum of p*b and p*(A p) over FLUID cells in a grid-stride loop, four
+// cells per trip whose loads (content and p at the cell and its six neighbours, b)
+// are all issued before any is used; each thread adds its cells in stride order.
 __global__ void k_nn_alpha_part(aqf::Geom G, const uint8_t* __restrict__ content, const float* __restrict__ rhs,
                                 const float* __restrict__ p, double* __restrict__ part) {
   __shared__ double s1[256], s2[256];
   double qb = 0, qq = 0;
-  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < G.ncells_hash;
-       n += (long long)gridDim.x * blockDim.x) {
-    int x, y, z;
-    cell3(G, n, x, y, z);
-    const long long gi = G.g(x, y, z);
-    if (content[gi] != aqf::FLUID) continue;
-    const double pc = (double)p[gi];
-    double ap = 6.0 * pc;
-    const bool in[6] = {x + 1 < G.X, x > 0, y + 1 < G.Y, y > 0, z + 1 < G.Z, z > 0};
-    const long long o[6] = {G.sx, -G.sx, (long long)G.EZ, -(long long)G.EZ, 1, -1};
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long o[6] = {G.sx, -G.sx, (long long)G.EZ, -(long long)G.EZ, 1, -1};
+  for (long long n0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; n0 < G.ncells_hash; n0 += 4 * stride) {
+    uint8_t cc[4], cn[4][6];
+    float pc[4], pn[4][6], bc[4];
+    bool ok[4], in[4][6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      if (!in[k])
-        ap -= pc;
-      else if (content[gi + o[k]] == aqf::FLUID)
-        ap -= (double)p[gi + o[k]];
+    for (int k = 0; k < 4; ++k) {
+      const long long n = n0 + k * stride;
+      ok[k] = n < G.ncells_hash;
+      int x, y, z;
+      cell3(G, ok[k] ? n : 0, x, y, z);
+      const long long gi = G.g(x, y, z);
+      in[k][0] = x + 1 < G.X, in[k][1] = x > 0, in[k][2] = y + 1 < G.Y;
+      in[k][3] = y > 0, in[k][4] = z + 1 < G.Z, in[k][5] = z > 0;
+      cc[k] = content[gi];
+      pc[k] = p[gi];
+      bc[k] = rhs[gi];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const long long gn = in[k][j] ? gi + o[j] : gi;
+        cn[k][j] = content[gn];
+        pn[k][j] = p[gn];
+      }
     }
-    qb += pc * (double)rhs[gi];
-    qq += pc * ap;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (!ok[k] || cc[k] != aqf::FLUID) continue;
+      const double q = (double)pc[k];
+      double ap = 6.0 * q;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        if (!in[k][j])
+          ap -= q;
+        else if (cn[k][j] == aqf::FLUID)
+          ap -= (double)pn[k][j];
+      }
+      qb += q * (double)bc[k];
+      qq += q * ap;
+    }
   }
   s1[threadIdx.x] = qb;
   s2[threadIdx.x] = qq;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
-      s1[threadIdx.x] += s1[threadIdx.x + o];
-      s2[threadIdx.x] += s2[threadIdx.x + o];
+  for (int r = 128; r > 0; r >>= 1) {
+    if ((int)threadIdx.x < r) {
+      s1[threadIdx.x] += s1[threadIdx.x + r];
+      s2[threadIdx.x] += s2[threadIdx.x + r];
     }
     __syncthreads();
   }

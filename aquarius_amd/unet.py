@@ -95,6 +95,10 @@ class UNet:
         assert g.size == self.param_count
         check(self.lib.aq_unet_adam_step_host(self.h, _fp(g)))
 
+    def set_lr(self, lr: float) -> None:
+        check(self.lib.aq_unet_set_lr(self.h, float(lr)))
+        self.cfg.lr = float(lr)
+
     def adam_state(self):
         """(m, v, t): the Adam moments and the step count."""
         m = np.empty(self.param_count, np.float32)

@@ -46,19 +46,37 @@ def random_scene(rng: np.random.Generator, n: int = 64, seed: int = 0):
     return cfg
 
 
-def converge(h: FlipHandle, tol: float, cap: int, check_every: int = 100, start=None):
+def converge(h: FlipHandle, tol: float, cap: int, check_every: int = 100, start=None,
+             stall_sweeps: int = 0):
     """Re-solve the handle's current pressure system with plain Jacobi, from zero or
-    from `start` (a pressure field of the same extent)."""
+    from `start` (a pressure field of the same extent). With stall_sweeps > 0 the solve
+    also ends when that many sweeps improve the residual by less than 1 % (FP32
+    Jacobi's round-off floor above tol)."""
     h.set_field("pressure", np.zeros(h.field_shape("pressure"), np.float32) if start is None else start)
-    return h.solve_pressure(tol, cap, check_every)
+    if stall_sweeps <= 0:
+        return h.solve_pressure(tol, cap, check_every)
+    total, best, flat = 0, np.inf, 0
+    while True:
+        it, rel, ok = h.solve_pressure(tol, min(stall_sweeps, cap - total), check_every)
+        total += it
+        if ok or total >= cap or not np.isfinite(rel):
+            return total, rel, ok
+        if rel < 0.99 * best:
+            best, flat = rel, 0
+        else:
+            flat += 1
+            if flat >= 2:
+                return total, rel, False
 
 
 def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int = 4,
              max_step: int = 60, tol: float = 1e-5, accept: float = 1e-4, cap: int = 60000,
-             log=print, solver: str = "jacobi"):
+             log=print, solver: str = "jacobi", stall_sweeps: int = 0):
     """Fill the network's device dataset with `pairs` converged systems. Within a
     scene each system is solved starting from the previous stop's converged pressure
-    (the fixed point does not depend on the start, only the sweep count does)."""
+    (the fixed point does not depend on the start, only the sweep count does). With
+    stall_sweeps > 0 a Jacobi solve that stalls at its round-off floor above tol ends
+    there (and is kept if its residual is <= accept)."""
     net.reserve(pairs, (n, n, n))
     rng = np.random.default_rng(seed)
     scene = 0
@@ -76,7 +94,7 @@ def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int =
             if solver == "mgpcg":
                 it, rel, _ = h.solve_pressure_mgpcg(1e-10, max_iters=300)
             else:
-                it, rel, _ = converge(h, tol, cap, start=prev)
+                it, rel, _ = converge(h, tol, cap, start=prev, stall_sweeps=stall_sweeps)
             sweeps += it
             if rel <= accept and net.dataset_size < pairs:
                 net.add_sample(h)
@@ -169,12 +187,20 @@ def main(argv=None) -> int:
     t.add_argument("--steps", type=int, default=3000)
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--schedule", choices=("constant", "cosine"), default="constant",
+                   help="Adam step size: constant, or cosine decay from --lr to --lr-min")
+    t.add_argument("--lr-min", type=float, default=1e-5)
+    t.add_argument("--keep-best", action="store_true",
+                   help="keep the parameters with the lowest held-out loss seen (checked every 250 steps)")
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--out", type=Path, default=WEIGHTS)
     t.add_argument("--eval-scenes", type=int, default=8)
     t.add_argument("--tol", type=float, default=1e-5)
     t.add_argument("--eval-128", action="store_true")
     t.add_argument("--mgpcg", action="store_true", help="converge the training systems with MG-PCG")
+    t.add_argument("--per-scene", type=int, default=4, help="systems captured per random scene")
+    t.add_argument("--stall-sweeps", type=int, default=0,
+                   help="end a Jacobi solve after this many sweeps without a 1%% improvement (0: never)")
     c = sub.add_parser("compare")
     c.add_argument("--weights", type=Path, default=WEIGHTS)
     c.add_argument("--n", type=int, default=64)
@@ -187,19 +213,29 @@ def main(argv=None) -> int:
         net = UNet(lr=a.lr, seed=a.seed + 1)
         total = a.pairs + a.held_out
         print(f"[warmstart] generating {total} pairs at {a.n}^3", flush=True)
-        gen = generate(net, total, n=a.n, seed=a.seed, tol=a.tol, solver="mgpcg" if a.mgpcg else "jacobi")
+        gen = generate(net, total, n=a.n, seed=a.seed, tol=a.tol, solver="mgpcg" if a.mgpcg else "jacobi",
+                       per_scene=a.per_scene, stall_sweeps=a.stall_sweeps)
         print(f"[warmstart] data: {json.dumps(gen)}", flush=True)
         train_n = a.pairs
         l0 = net.eval(train_n, a.held_out)
         t0 = time.time()
         hist = []
         chunk = 250
+        best = (np.inf, None, 0)
         for s in range(0, a.steps, chunk):
             k = min(chunk, a.steps - s)
+            if a.schedule == "cosine":  # from lr down to lr_min over the run, per chunk
+                net.set_lr(a.lr_min + 0.5 * (a.lr - a.lr_min) * (1 + np.cos(np.pi * s / a.steps)))
             h = net.train(k, a.batch, train_n, seed=a.seed * 7919 + s)
             hist.append(float(h.mean()))
-            print(f"[warmstart] step {s + k}: train loss {hist[-1]:.4e} "
-                  f"held-out {net.eval(train_n, a.held_out):.4e} ({time.time() - t0:.0f} s)", flush=True)
+            ho = net.eval(train_n, a.held_out)
+            if a.keep_best and ho < best[0]:
+                best = (ho, net.get_params(), s + k)
+            print(f"[warmstart] step {s + k}: lr {net.cfg.lr:.2e} train loss {hist[-1]:.4e} "
+                  f"held-out {ho:.4e} ({time.time() - t0:.0f} s)", flush=True)
+        if a.keep_best and best[1] is not None:
+            net.set_params(best[1])
+            print(f"[warmstart] kept the parameters of step {best[2]} (held-out {best[0]:.4e})", flush=True)
         l1 = net.eval(train_n, a.held_out)
         a.out.parent.mkdir(parents=True, exist_ok=True)
         net.save(a.out)

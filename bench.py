@@ -23,9 +23,10 @@ Prints one JSON line with the contract fields plus:
                 against the U-Net guess alone, the guess with corrections, and
                 MG-PCG, over steps 2-11 (ball in free fall) and again over steps
                 162-171 (after the ball has met the pool)
-  fp64_error    one step of the workload on the GPU (FP32) against the FP64 CPU
-                oracle from the same seeded particles: relative L2 of pressure, face
-                velocities and particle positions and velocities
+  fp64_error    step 111 of the workload on the GPU (FP32) against the FP64 CPU oracle
+                stepping once from the GPU's state after step 110: relative L2 of
+                pressure, face velocities, fractions and particle positions,
+                velocities and fractions
   paired_diffusion  diffusion sweeps paired with pressure sweeps per step, how many of
                 them computed their diffusion part in the profiled pass (a sweep after
                 one that changed no bit skips the identical work), and the steps/s of
@@ -296,28 +297,84 @@ def solve_comparison(cfg, at_step: int, cap: int) -> dict:
     return solve
 
 
-def fp64_error(cfg) -> dict:
-    """One step on the GPU (FP32) against one step of the FP64 CPU oracle from the
-    same seeded particles: relative L2 differences. Particles are matched through the
-    sort permutations (both runs start from the same particle order)."""
+# FLOP of one U-Net forward pass per voxel of its finest level: 2 * 27 * cin * cout per
+# 3x3x3 convolution at each level (the finest level's voxels, 1/8 and 1/64 of them
+# below), plus the 1x1x1 head.
+UNET_LEVEL_CONVS = {1: [(8, 16), (16, 16), (48, 16), (16, 16)], 2: [(16, 32), (32, 32), (96, 32), (32, 32)],
+                    3: [(32, 64), (64, 64)]}
+
+
+def unet_flop_per_voxel() -> float:
+    f = 0.0
+    for lvl, convs in UNET_LEVEL_CONVS.items():
+        f += sum(2 * 27 * ci * co for ci, co in convs) / 8 ** (lvl - 1)
+    return f + 2 * 16
+
+
+def unet_inference(cfg, at_step: int, reps: int = 50) -> dict:
+    """U-Net inference on the pressure system of step `at_step` of a fresh handle: the
+    guess (restriction to the network's 64^3 resolution, forward pass, prolongation,
+    energy-norm scaling), timed over `reps` synchronised calls, with the FLOP rate of
+    the forward pass's convolutions against the BF16 dense MFMA peak."""
+    from aquarius_amd.fluid import FlipHandle
+    from aquarius_amd.unet import nn_guess
+    net = load_warmstart_net()
+    if net is None:
+        return {"unet": None}
+    h = FlipHandle(cfg)
+    h.steps(at_step)
+    for _ in range(3):
+        nn_guess(h, net)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nn_guess(h, net)
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    h.close()
+    X, Y, Z = cfg.size
+    n = max(X, Y, Z)
+    vox = X * Y * Z
+    while n > 64:
+        n //= 2
+        vox //= 8
+    gflop = unet_flop_per_voxel() * vox / 1e9
+    return {"unet_guess_ms": round(ms, 4), "network_voxels": vox, "forward_gflop": round(gflop, 2),
+            "tflops_whole_guess": round(gflop / ms, 2), "peak_bf16_tflops": 2500.0,
+            "frac_of_peak_whole_guess": round(gflop / ms / 2500.0, 4)}
+
+
+STATE_FIELDS = ("content", "velocity", "new_velocity", "pressure", "divergence", "density", "particle_count",
+                "has_velocity", "volume_fractions", "new_volume_fractions")
+
+
+def fp64_error(cfg, at_step: int) -> dict:
+    """The FP32 GPU step against an FP64 run of the same step: a fresh handle runs
+    `at_step` steps, its whole state (particles in their order and every grid field)
+    is handed to the FP64 CPU oracle, then both run one step. Relative L2 differences;
+    particles are matched through the sort permutations (both start from the same
+    order)."""
     import numpy as np
 
     from aquarius_amd.fluid import FlipHandle
     from oracle import binding as ob
     ob.set_threads(cpu_threads())
     g = FlipHandle(cfg)
-    o = ob.Oracle(cfg, fp64=True)
+    g.steps(at_step)
+    pos, vel, frac = g.get_particles()
+    o = ob.Oracle(cfg, fp64=True, pos=pos, vel=vel, frac4=frac, init=False)
+    for f in STATE_FIELDS:
+        o.set_field(f, g.get_field(f))
     g.step()
     o.step()
 
     def rel(a, b):
         a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
         return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-    out = {"reference": "CPU oracle in FP64 (same kernels' logic), one step from the same particles"}
-    for f in ("pressure", "new_velocity", "velocity"):
+    out = {"reference": (f"CPU oracle in FP64 (the same kernels' logic) stepping once from the GPU state "
+                         f"after step {at_step}")}
+    for f in ("pressure", "new_velocity", "velocity", "new_volume_fractions"):
         out[f] = rel(g.get_field(f), o.get_field(f))
-    gp, gv, _ = g.get_particles()
-    op, ov, _ = o.get_particles()
+    gp, gv, gf = g.get_particles()
+    op, ov, of = o.get_particles()
     _, _, _, pg = g.get_cells()
     _, _, _, po = o.get_cells()
     ig, io = np.empty_like(pg), np.empty_like(po)
@@ -325,10 +382,11 @@ def fp64_error(cfg) -> dict:
     io[po] = np.arange(po.size)
     out["particle_position"] = rel(gp[ig], op[io])
     out["particle_velocity"] = rel(gv[ig], ov[io])
+    out["particle_fractions"] = rel(gf[ig], of[io])
     out["sort_slots_differing"] = int((pg != po).sum())
     g.close()
     o.close()
-    return {k: (round(v, 12) if isinstance(v, float) else v) for k, v in out.items()}
+    return {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in out.items()}
 
 
 def cpu_threads() -> int:
@@ -403,6 +461,8 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--timed-only", action="store_true",
                     help="warm-up, timed region and profiled pass only (for kernel traces)")
+    ap.add_argument("--unet-only", action="store_true",
+                    help="time the U-Net inference on the workload's step-110 system only")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -416,6 +476,9 @@ def main() -> int:
     from aquarius_amd.fluid import FlipHandle
 
     cfg = configs.SCENES[args.scene]()
+    if args.unet_only:
+        print(json.dumps(unet_inference(cfg, args.solve_step)), flush=True)
+        return 0
     h = FlipHandle(cfg)
     n = h.particle_count
     _, _, frac = h.get_particles()
@@ -491,7 +554,8 @@ def main() -> int:
                         "after_contact": tolerance_mode_steps(cfg, args.tol_steps, args.solve_cap,
                                                               advance=args.contact_step)}
     if not args.no_fp64 and not args.timed_only:
-        fp64 = fp64_error(cfg)
+        fp64 = fp64_error(cfg, args.solve_step)
+    unet = None if args.no_solve or args.timed_only else unet_inference(cfg, args.solve_step)
 
     out = {
         "metric": METRIC,
@@ -520,6 +584,7 @@ def main() -> int:
         "jacobi_to_tol": solve,
         "tolerance_mode": tol_mode,
         "fp64_error": fp64,
+        "unet_inference": unet,
         "profiled_pass_ms_per_step": round(eager_ms_per_step, 4),
     }
     if not args.no_cpu_baseline and not args.timed_only:

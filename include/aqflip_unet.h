@@ -66,6 +66,8 @@ aq_status aq_unet_gradients_host(aq_unet* net, const float* x, const float* targ
  * on the FP32 master parameters with the given gradients (param_count floats), the
  * update aq_unet_train applies after each batch's backward pass. */
 aq_status aq_unet_adam_step_host(aq_unet* net, const float* grads);
+/* The Adam step size for the following steps (a schedule is a sequence of calls). */
+aq_status aq_unet_set_lr(aq_unet* net, float lr);
 /* The Adam moments m, v (param_count floats each; either may be NULL) and the step
  * count t. */
 aq_status aq_unet_get_adam_state(aq_unet* net, float* m, float* v, int64_t* t);

@@ -133,6 +133,44 @@ def test_forward_matches_oracle():
 
 
 @pytest.mark.gpu
+def test_tiled_forward_matches_oracle(monkeypatch):
+    """Extents that divide into the LDS-tiled convolution's tiles (W % 16, H % 8): the
+    32 x 16 x 32 and 16 x 8 x 16 levels run k_conv3_tile, the 8 x 4 x 8 level k_conv3.
+    How far a bf16 network lands from the bf16-emulating oracle depends on the input
+    (rounding flips of activations near ReLU and bf16 boundaries propagate): 2.9e-3 for
+    both kernels here against 2e-4 on test_forward_matches_oracle's input, so the tiled
+    kernel is held to the gather kernel's own error on the same input."""
+    from aquarius_amd.unet import UNet
+    net = UNet(seed=17)
+    x, _, _ = random_problem(B=1, D=32, H=16, W=32, seed=5)
+    want = U.forward(net.get_params(), x, bf=True)
+    err = rel_l2(net.forward(x), want)
+    monkeypatch.setenv("AQUNET_NO_TILE", "1")
+    err_gather = rel_l2(net.forward(x), want)
+    print(f"tiled forward rel_l2 vs bf16 oracle {err:.2e} (gather kernel {err_gather:.2e})")
+    assert err < 1e-2 and err <= 1.2 * err_gather + 2e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(8, 64, 64, 64), (1, 32, 32, 32)])
+def test_tiled_forward_matches_untiled(monkeypatch, shape):
+    """The LDS-tiled forward convolution against the gather kernel on the same weights and
+    input, at the shapes that select each of its variants (tile depth 1 or 2, one or two
+    16-channel output tiles per block): the same MFMA products, summed in another k
+    order only where a layer's input is staged in channel chunks, so the outputs agree
+    to bf16 rounding flips."""
+    from aquarius_amd.unet import UNet
+    net = UNet(seed=19)
+    x, _, _ = random_problem(*shape, seed=6)
+    y_tile = net.forward(x)
+    monkeypatch.setenv("AQUNET_NO_TILE", "1")
+    y_gather = net.forward(x)
+    err = rel_l2(y_tile, y_gather)
+    print(f"{shape}: tiled vs gather rel_l2 {err:.2e}")
+    assert err < 3e-3
+
+
+@pytest.mark.gpu
 def test_gradients_match_oracle():
     from aquarius_amd.unet import UNet
     net = UNet(seed=11)
