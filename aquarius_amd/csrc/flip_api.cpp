@@ -680,20 +680,25 @@ aq_status sweep_to_tol(aq_flip* h, int budget, bool detect_stall, int* sweeps, d
 
 // Refinement of a U-Net guess already in the pressure buffer: Jacobi to the solver's
 // tol within `budget` sweeps, the network correcting the iterate from its residual
-// every nn_refresh sweeps (0: never). A correction is judged after the sweeps that
-// follow it: the network removes smooth error that Jacobi is slow on while adding
-// high-frequency residual that Jacobi removes quickly, so the residual right after a
-// correction says little. If the residual after those sweeps is not below the one
-// before the correction, the iterate from before the correction is restored bit for
-// bit and Jacobi continues alone (near FP32 Jacobi's round-off floor the network's
-// output is mostly noise).
+// every nn_refresh sweeps (0: never). While the residual at the end of each chunk of
+// sweeps keeps falling by at least 1 % from the previous chunk's, every correction is
+// kept: the network removes smooth error that Jacobi is slow on while adding
+// high-frequency residual that the following sweeps remove, so the residual right after
+// a correction, or even one chunk later on a large grid, says little. Once a chunk
+// ends within 1 % of the previous one (Jacobi near its FP32 round-off floor, where the
+// network's output is mostly noise), the next correction is judged by the residual
+// after the sweeps that follow it: if that is not below the residual before the
+// correction, the iterate from before it is restored bit for bit and Jacobi continues
+// without further corrections.
 aq_status nn_refine(aq_flip* h, aq_unet* net, int budget, int nn_refresh, int* total, double* res,
                     bool* stalled, bool* fallback, int64_t* corrections, int64_t* rejected) {
   const size_t bytes = (size_t)h->G.ncell * sizeof(float);
   int left = budget, sweeps = 0;
   bool correcting = nn_refresh > 0;
-  bool judging = false;  // the last chunk followed a correction
+  bool judging = false;  // the last chunk followed a correction that is being judged
   double before = 0;     // residual of the iterate saved in p_keep
+  double last = 0;       // residual at the end of the previous chunk
+  bool have_last = false;
   int pb_keep = 0;
   bool clean_keep = false;
   *total = 0;
@@ -702,6 +707,9 @@ aq_status nn_refine(aq_flip* h, aq_unet* net, int budget, int nn_refresh, int* t
     AQ_TRY(sweep_to_tol(h, chunk, true, &sweeps, res, stalled));
     *total += sweeps;
     left -= sweeps;
+    const bool flat = have_last && !(*res < 0.99 * last);
+    last = *res;
+    have_last = true;
     if (judging) {
       judging = false;
       if (!(*res <= h->solver.tol) && !(*res < before)) {
@@ -718,14 +726,16 @@ aq_status nn_refine(aq_flip* h, aq_unet* net, int budget, int nn_refresh, int* t
     }
     if (!std::isfinite(*res)) *fallback = true;
     if (*fallback || *res <= h->solver.tol || left <= 0 || *stalled || !correcting) break;
-    if (!h->p_keep) AQ_TRY(h->allocT(&h->p_keep, (size_t)h->G.ncell));
-    pb_keep = h->pb;
-    clean_keep = h->air_clean[h->pb];
-    HIP_TRY(hipMemcpyAsync(h->p_keep, h->gr.p[h->pb], bytes, hipMemcpyDeviceToDevice, h->stream));
+    if (flat) {  // keep a copy of the iterate: this correction will be judged
+      if (!h->p_keep) AQ_TRY(h->allocT(&h->p_keep, (size_t)h->G.ncell));
+      pb_keep = h->pb;
+      clean_keep = h->air_clean[h->pb];
+      HIP_TRY(hipMemcpyAsync(h->p_keep, h->gr.p[h->pb], bytes, hipMemcpyDeviceToDevice, h->stream));
+    }
     before = *res;
     AQ_TRY(aq_flip_nn_correct(h, net));
     *corrections += 1;
-    judging = true;
+    judging = flat;
   }
   return AQ_OK;
 }

@@ -101,8 +101,10 @@ aq_status aq_flip_nn_correct(aq_flip* flip, aq_unet* net);
 /* The AQ_SOLVE_TOL_NN solve on the handle's current system, outside a step: the U-Net
  * guess, then Jacobi to tol (FP64 relative residual checked every check_every sweeps,
  * at most max_iters sweeps, stalls detected as the solver's stall_checks says) with a
- * U-Net correction every nn_refresh sweeps (0: none). A correction is judged by the
- * residual after the sweeps that follow it; if that is not below the residual before
+ * U-Net correction every nn_refresh sweeps (0: none). Corrections are kept while each
+ * chunk of nn_refresh sweeps ends at least 1 % below the previous one. After a chunk
+ * that does not (Jacobi near its round-off floor), the next correction is judged by the
+ * residual after the sweeps that follow it: if that is not below the residual before
  * the correction, the iterate from before it is restored bit for bit and Jacobi
  * continues without further corrections. Outputs may be NULL: sweeps, the final
  * residual, corrections applied and corrections undone. AQ_ESTALL when tol was not
@@ -137,9 +139,9 @@ typedef struct aq_flip_solver_stats {
   double last_guess_residual; /* residual of the last initial guess */
   int32_t last_fallback;
   int64_t nn_corrections; /* U-Net residual corrections applied */
-  int64_t nn_rejected;    /* of those, undone: the residual after the sweeps that
-                             followed was not below the one before the correction
-                             (the solve then continues with Jacobi alone) */
+  int64_t nn_rejected;    /* of those, undone (see aq_flip_solve_pressure_nn: judged
+                             near the round-off floor, the residual after the sweeps
+                             that followed was not below the one before it) */
 } aq_flip_solver_stats;
 /* Select how aq_flip_step solves for pressure; net is required for
  * AQ_SOLVE_TOL_NN and must outlive its use. */

