@@ -101,6 +101,8 @@ struct Bins {
   int* hash;      // per particle, previous order
   int* rank;      // per particle: slot within its cell's segment (pre-sort)
   int* perm;      // sorted slot -> previous index
+  uint8_t* multi; // X*Y*Z: 1 when a cell received more than one run of ranks (its segment
+                  // may be out of order); k_segsort sorts those and clears every flag
 };
 
 struct StepParams {

@@ -339,6 +339,7 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   AQ_TRY(h->allocT(&h->B.hash, (size_t)Np));
   AQ_TRY(h->allocT(&h->B.rank, (size_t)Np));
   AQ_TRY(h->allocT(&h->B.perm, (size_t)Np));
+  AQ_TRY(h->allocT(&h->B.multi, (size_t)nch));  // zeroed here; k_segsort clears what it reads
   h->nparts_red = 512;  // volume statistics: 512 partial rows of 16 columns
   // (the residual uses two columns per sweep chunk)
   AQ_TRY(h->allocT(&h->red_partial, std::max<size_t>((size_t)h->nparts_red * 16, 2 * (size_t)sweep_chunks(G))));

@@ -99,8 +99,10 @@ __device__ __forceinline__ int cell_hash(const Geom& G, float px, float py, floa
 // before the segment sort. The groups are found first (ballots and shuffles only),
 // then every group leader issues its atomic in the same instruction, so a wave
 // waits for one atomic round trip however many cells its particles span.
+// A group whose reservation does not start the cell's segment (base > 0) marks the cell:
+// only cells with two or more runs can be out of order.
 __device__ __forceinline__ void bin_particle(int hh, bool valid, int* __restrict__ count,
-                                             int& rank) {
+                                             uint8_t* __restrict__ multi, int& rank) {
   const int lane = (int)(threadIdx.x & 63);
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long pending = __ballot(valid);
@@ -118,13 +120,16 @@ __device__ __forceinline__ void bin_particle(int hh, bool valid, int* __restrict
     pending &= ~same;
   }
   int base = 0;
-  if (valid && lane == leader) base = atomicAdd(&count[hh], __popcll(group));
+  if (valid && lane == leader) {
+    base = atomicAdd(&count[hh], __popcll(group));
+    if (base > 0) multi[hh] = 1;
+  }
   base = __shfl(base, leader, 64);
   rank = base + __popcll(group & lt);
 }
 
 __global__ void k_hash_bin(Geom G, Particles P, long long Np, int* __restrict__ hash,
-                           int* __restrict__ rank, int* __restrict__ count) {
+                           int* __restrict__ rank, int* __restrict__ count, uint8_t* __restrict__ multi) {
   const int i = gtid();
   const bool valid = i < Np;
   int hh = 0;
@@ -133,7 +138,7 @@ __global__ void k_hash_bin(Geom G, Particles P, long long Np, int* __restrict__ 
     hh = cell_hash(G, a.x, a.y, a.z);
   }
   int r;
-  bin_particle(hh, valid, count, r);
+  bin_particle(hh, valid, count, multi, r);
   if (valid) {
     hash[i] = hh;
     rank[i] = r;
@@ -141,7 +146,7 @@ __global__ void k_hash_bin(Geom G, Particles P, long long Np, int* __restrict__ 
 }
 void launch_hash_count(const Geom& G, Particles P, long long Np, Bins B, hipStream_t s) {
   hipLaunchKernelGGL(k_hash_bin, dim3(nblocks(Np)), dim3(kBlock), 0, s, G, P, Np, B.hash, B.rank,
-                     B.count);
+                     B.count, B.multi);
 }
 
 // ------------------------------------------------------------------ scan
@@ -277,13 +282,17 @@ __global__ void k_scatter(const int* __restrict__ hash, const int* __restrict__ 
   perm[offsets[hash[i]] + rank[i]] = i;
 }
 
-// Segments of up to kSegFast entries are checked from registers loaded in one
-// round trip (predicated loads); longer ones by a loop. An unsorted segment is
-// insertion-sorted in place.
+// Only cells that received two or more runs of ranks (flagged by bin_particle) can be
+// out of order; every other thread reads its flag and returns, and every flag is
+// cleared for the next binning. Segments of up to kSegFast entries are checked from
+// registers loaded in one round trip (predicated loads); longer ones by a loop. An
+// unsorted segment is insertion-sorted in place.
 constexpr int kSegFast = 16;
-__global__ void k_segsort(const int* __restrict__ offsets, int ncells, int* __restrict__ perm) {
+__global__ void k_segsort(const int* __restrict__ offsets, int ncells, int* __restrict__ perm,
+                          uint8_t* __restrict__ multi) {
   const int c = gtid();
-  if (c >= ncells) return;
+  if (c >= ncells || !multi[c]) return;
+  multi[c] = 0;
   const int s = offsets[c], e = offsets[c + 1];
   bool sorted = true;
   if (e - s <= kSegFast) {
@@ -322,7 +331,7 @@ void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, lon
   hipLaunchKernelGGL(k_scatter, dim3(nblocks(Np)), dim3(kBlock), 0, s, B.hash, B.rank, Np,
                      B.offsets, B.perm);
   hipLaunchKernelGGL(k_segsort, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, B.offsets,
-                     G.ncells_hash, B.perm);
+                     G.ncells_hash, B.perm, B.multi);
   if (K > 2)
     hipLaunchKernelGGL(k_gather<4>, dim3(nblocks(Np)), dim3(kBlock), 0, s, cur, alt, Np, B.perm);
   else
@@ -334,7 +343,7 @@ void launch_scatter_sort_gather(const Geom& G, Particles cur, Particles alt, lon
 // positions are the first three floats of records of `stride` floats.
 __global__ void k_hash_records(Geom G, const float* __restrict__ pos, long long n, int stride,
                                int* __restrict__ hash, int* __restrict__ rank,
-                               int* __restrict__ count) {
+                               int* __restrict__ count, uint8_t* __restrict__ multi) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
   int hh = 0;
@@ -343,7 +352,7 @@ __global__ void k_hash_records(Geom G, const float* __restrict__ pos, long long 
     hh = cell_hash(G, p[0], p[1], p[2]);
   }
   int r;
-  bin_particle(hh, valid, count, r);
+  bin_particle(hh, valid, count, multi, r);
   if (valid) {
     hash[i] = hh;
     rank[i] = r;
@@ -351,13 +360,13 @@ __global__ void k_hash_records(Geom G, const float* __restrict__ pos, long long 
 }
 void launch_hash_records(const Geom& G, const float* pos, long long n, int stride, Bins B, hipStream_t s) {
   hipLaunchKernelGGL(k_hash_records, dim3(nblocks(n)), dim3(kBlock), 0, s, G, pos, n, stride, B.hash,
-                     B.rank, B.count);
+                     B.rank, B.count, B.multi);
 }
 void launch_scatter_segsort(const Geom& G, long long n, Bins B, hipStream_t s) {
   hipLaunchKernelGGL(k_scatter, dim3(nblocks(n)), dim3(kBlock), 0, s, B.hash, B.rank, n, B.offsets,
                      B.perm);
   hipLaunchKernelGGL(k_segsort, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, B.offsets,
-                     G.ncells_hash, B.perm);
+                     G.ncells_hash, B.perm, B.multi);
 }
 // findCellStartEndImpl (Fluid/Fluid_kernels.cuh:3-22) from the offsets: first and
 // last (inclusive) sorted slot of every cell, -1 for empty cells.
@@ -839,10 +848,12 @@ constexpr int kSweepCells = 2;
 constexpr int kSweepSpan = kSweepCells * kBlock;  // cells per chunk (one workgroup's work)
 // Workgroups of a listed sweep (multiples of kXCD): they stride over the active list.
 // Measured at 128^3 (1,441 of 4,096 chunks active): a pressure sweep is fastest with
-// 1024 (3.59 us; 2048: 3.79), a paired sweep with 1536-2048 (6.19 us; 1024: 6.24,
-// 512: 7.28).
+// 1024 (3.59 us; 2048: 3.79); a paired sweep that computes its diffusion part takes
+// 6.19 us with 1536-2048 workgroups, 6.24 with 1024 and 7.28 with 512, while one that
+// skips it runs the pressure sweep alone, 4.56 us with 1024 against 4.66 with 2048
+// (graph replay of the 128^3 multiphase workload: 794 against 786 steps/s).
 constexpr int kListedGrid = 1024;
-constexpr int kListedGridPaired = 2048;
+constexpr int kListedGridPaired = 1024;
 
 int sweep_chunks(const Geom& G) { return (G.ncells_hash + kSweepSpan - 1) / kSweepSpan; }
 
@@ -1581,7 +1592,8 @@ __global__ void k_advect(Geom G, Particles P, long long Np, Grid gr, int nv, flo
 template <int K>
 __global__ void k_g2p_advect_hash(Geom G, Particles P, long long Np, Grid gr, int nv, int nf,
                                   float flip, float dt, int* __restrict__ hash,
-                                  int* __restrict__ rank, int* __restrict__ count) {
+                                  int* __restrict__ rank, int* __restrict__ count,
+                                  uint8_t* __restrict__ multi) {
   const int i = gtid();
   const bool valid = i < Np;
   int hh = 0;
@@ -1595,7 +1607,7 @@ __global__ void k_g2p_advect_hash(Geom G, Particles P, long long Np, Grid gr, in
     hh = cell_hash(G, r.p[0], r.p[1], r.p[2]);
   }
   int rk;
-  bin_particle(hh, valid, count, rk);
+  bin_particle(hh, valid, count, multi, rk);
   if (valid) {
     hash[i] = hh;
     rank[i] = rk;
@@ -1620,10 +1632,10 @@ void launch_g2p_advect_hash(const Geom& G, Particles P, long long Np, int K, Gri
                             int nf, StepParams sp, Bins B, hipStream_t s) {
   const int nb = nblocks(Np);
   switch (K) {
-    case 1: hipLaunchKernelGGL(k_g2p_advect_hash<1>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count); break;
-    case 2: hipLaunchKernelGGL(k_g2p_advect_hash<2>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count); break;
-    case 3: hipLaunchKernelGGL(k_g2p_advect_hash<3>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count); break;
-    default: hipLaunchKernelGGL(k_g2p_advect_hash<4>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count); break;
+    case 1: hipLaunchKernelGGL(k_g2p_advect_hash<1>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count, B.multi); break;
+    case 2: hipLaunchKernelGGL(k_g2p_advect_hash<2>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count, B.multi); break;
+    case 3: hipLaunchKernelGGL(k_g2p_advect_hash<3>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count, B.multi); break;
+    default: hipLaunchKernelGGL(k_g2p_advect_hash<4>, dim3(nb), dim3(kBlock), 0, s, G, P, Np, gr, nv, nf, sp.flip, sp.dt, B.hash, B.rank, B.count, B.multi); break;
   }
 }
 

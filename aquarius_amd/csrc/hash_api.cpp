@@ -140,9 +140,9 @@ Geom hash_geom(int sx, int sy, int sz, float cell_size) {
 }
 
 // ---- the entry point that allocates nothing (aq_hash_sort_device_noalloc)
-// Its bins live in one file-scope device array: per-cell counts and offsets, the scan
-// partials, and per-record hash, rank and permutation, for at most kStaticCells cells
-// and kStaticRecords records (67 MB on every GPU the library runs on, held until the
+// Its bins live in one file-scope device array: per-cell counts, offsets and multi-run
+// flags, the scan partials, and per-record hash, rank and permutation, for at most kStaticCells cells
+// and kStaticRecords records (69 MB on every GPU the library runs on, held until the
 // process ends). In host code `scratch` names a host-side stand-in, so the array's
 // device address is looked up once per device with hipGetSymbolAddress and only that
 // address is used from then on.
@@ -156,7 +156,8 @@ constexpr long long kOffParts = kOffOffsets + align64(kStaticCells + 1);
 constexpr long long kOffHash = kOffParts + align64(kStaticParts);
 constexpr long long kOffRank = kOffHash + align64(kStaticRecords);
 constexpr long long kOffPerm = kOffRank + align64(kStaticRecords);
-constexpr long long kStaticInts = kOffPerm + align64(kStaticRecords);
+constexpr long long kOffMulti = kOffPerm + align64(kStaticRecords);  // kStaticCells bytes
+constexpr long long kStaticInts = kOffMulti + align64(kStaticCells / 4);
 
 }  // namespace
 
@@ -208,9 +209,15 @@ aq_status aq_hash_create(int32_t sx, int32_t sy, int32_t sz, float cell_size, in
   aq_status st = h->alloc(&h->B.count, (size_t)nch, h->fixed);
   if (st == AQ_OK) st = h->alloc(&h->B.offsets, (size_t)nch + 1, h->fixed);
   if (st == AQ_OK) st = h->alloc(&h->B.partials, (size_t)(nch + 4095) / 4096 + 1, h->fixed);
+  if (st == AQ_OK) st = h->alloc(&h->B.multi, (size_t)nch, h->fixed);
+  // the multi-run flags start at zero; k_segsort clears each flag it reads
+  if (st == AQ_OK && hipMemsetAsync(h->B.multi, 0, (size_t)nch, h->stream) != hipSuccess)
+    st = hfail(AQ_EHIP, "clearing the bins failed");
   if (st == AQ_OK) st = h->alloc(&h->d_start, (size_t)nch, h->fixed);
   if (st == AQ_OK) st = h->alloc(&h->d_end, (size_t)nch, h->fixed);
   if (st == AQ_OK) st = h->ensure(std::max<int64_t>(capacity, 1));
+  // the clear must be complete before a sort queued on a caller's stream reads the flags
+  if (st == AQ_OK && hipStreamSynchronize(h->stream) != hipSuccess) st = hfail(AQ_EHIP, "stream synchronisation failed");
   if (st != AQ_OK) return bail(st);
   *out = h;
   return AQ_OK;
@@ -290,9 +297,12 @@ aq_status aq_hash_sort_device_noalloc(int32_t sx, int32_t sy, int32_t sz, float 
   B.hash = base + kOffHash;
   B.rank = base + kOffRank;
   B.perm = base + kOffPerm;
+  B.multi = reinterpret_cast<uint8_t*>(base + kOffMulti);
   const hipStream_t s = (hipStream_t)stream;
-  // The same kernels as aq_hash_sort_device; the counts start from a fill kernel.
+  // The same kernels as aq_hash_sort_device; the counts and the multi-run flags start
+  // from fill kernels (a previous call that failed part-way may have left flags set).
   launch_fill_u32(B.count, 0u, G.ncells_hash, s);
+  launch_fill_u32(B.multi, 0u, (G.ncells_hash + 3) / 4, s);
   if (n > 0) launch_hash_records(G, d_rec, n, stride, B, s);
   launch_scan(G, B, s);
   if (n > 0) launch_scatter_segsort(G, n, B, s);

@@ -375,15 +375,16 @@ def fp64_error(cfg, at_step: int) -> dict:
         out[f] = rel(g.get_field(f), o.get_field(f))
     gp, gv, gf = g.get_particles()
     op, ov, of = o.get_particles()
-    _, _, _, pg = g.get_cells()
-    _, _, _, po = o.get_cells()
+    _, _, gh, pg = g.get_cells()
+    _, _, oh, po = o.get_cells()
     ig, io = np.empty_like(pg), np.empty_like(po)
     ig[pg] = np.arange(pg.size)
     io[po] = np.arange(po.size)
     out["particle_position"] = rel(gp[ig], op[io])
     out["particle_velocity"] = rel(gv[ig], ov[io])
     out["particle_fractions"] = rel(gf[ig], of[io])
-    out["sort_slots_differing"] = int((pg != po).sum())
+    # particles that the FP32 and FP64 positions put in different cells
+    out["particles_in_other_cells"] = int((gh[ig] != oh[io]).sum())
     g.close()
     o.close()
     return {k: (float(f"{v:.4g}") if isinstance(v, float) else v) for k, v in out.items()}

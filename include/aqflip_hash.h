@@ -70,7 +70,7 @@ aq_status aq_hash_sort_device(aq_hash* h, const float* d_records, int64_t n, int
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc): aq_hash_sort_device without a handle, calling no hipMalloc/hipFree.
  * Its bins (per-cell counts and offsets, per-record hash, rank and permutation) live
- * in one static device array of 67 MB that every GPU the library runs on holds until
+ * in one static device array of 69 MB that every GPU the library runs on holds until
  * the process ends, so it takes at most 2,097,152 cells (size_x*size_y*size_z) and
  * 4,194,304 records; larger requests fail with AQ_EINVAL. Inputs, outputs, cell_count
  * and stream ordering are those of aq_hash_sort_device. Every call on one GPU shares
