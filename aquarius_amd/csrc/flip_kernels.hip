@@ -207,11 +207,28 @@ __global__ void k_scan_reduce(const int* __restrict__ count, int n, int* __restr
   if (threadIdx.x == 0) partials[b] = tot;
 }
 
+// Exclusive scan of the block totals in place, by one block walking them 256 at a time
+// (grids with more than kScanDirectParts blocks, where every final block re-summing the
+// totals before it would grow with the square of the block count).
+constexpr int kScanDirectParts = 1024;
+__global__ void k_scan_parts(int* __restrict__ partials, int nparts) {
+  int carry = 0;
+  for (int j0 = 0; j0 < nparts; j0 += kBlock) {
+    const int j = j0 + (int)threadIdx.x;
+    const int v = j < nparts ? partials[j] : 0;
+    int tot;
+    const int ex = block_exclusive_scan(v, &tot);
+    if (j < nparts) partials[j] = carry + ex;
+    carry += tot;
+  }
+}
+
 // Block b adds the totals of blocks 0..b-1 (read here, with its counts, in the same
-// round trip) to its own exclusive scan; the last block also writes offsets[n].
-// Integer sums, so any association gives the same offsets.
+// round trip; or, when `prefixed`, their sum already formed by k_scan_parts) to its own
+// exclusive scan; the last block also writes offsets[n]. Integer sums, so any
+// association gives the same offsets.
 __global__ void k_scan_final(const int* __restrict__ count, int n, const int* __restrict__ partials,
-                             int nparts, int* __restrict__ offsets) {
+                             int nparts, int* __restrict__ offsets, int prefixed) {
   __shared__ int wsum[kBlock / 64], wpre[kBlock / 64];
   const int b = blockIdx.x;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -222,7 +239,10 @@ __global__ void k_scan_final(const int* __restrict__ count, int n, const int* __
 #pragma unroll
   for (int k = 0; k < kScanPerThread; ++k) s += v[k];
   int pre = 0;
-  for (int j = threadIdx.x; j < b; j += kBlock) pre += partials[j];
+  if (!prefixed)
+    for (int j = threadIdx.x; j < b; j += kBlock) pre += partials[j];
+  else if (threadIdx.x == 0)
+    pre = partials[b];  // the other threads add 0
   int x = s;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -264,8 +284,10 @@ void launch_scan(const Geom& G, Bins B, hipStream_t s) {
   const int n = G.ncells_hash;
   const int nparts = (n + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(kBlock), 0, s, B.count, n, B.partials);
+  const int prefixed = nparts > kScanDirectParts ? 1 : 0;
+  if (prefixed) hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kBlock), 0, s, B.partials, nparts);
   hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, s, B.count, n, B.partials, nparts,
-                     B.offsets);
+                     B.offsets, prefixed);
 }
 
 // ------------------------------------------------------------------ stable counting sort

@@ -164,3 +164,13 @@ def test_cell_count_below_the_grid_is_rejected():
     with pytest.raises(Exception, match="cell_count"):
         sh.sort(np.zeros((1, 3), np.float32), cell_count=10)
     sh.close()
+
+
+@pytest.mark.gpu
+def test_gpu_grid_with_more_than_1024_scan_blocks():
+    """256 x 256 x 80 cells: 1,280 scan blocks of 4,096 cells, above the 1,024 for which
+    each final block sums the totals before it, so the block totals are scanned by
+    k_scan_parts first."""
+    rng = np.random.default_rng(11)
+    pos = rng.uniform(0, 1, (1_500_000, 3)).astype(np.float32) * np.array([256, 256, 80], np.float32) * 0.5
+    gpu_case((256, 256, 80), 0.5, pos)
