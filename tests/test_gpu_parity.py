@@ -217,6 +217,43 @@ def test_fp64_error_statement_16():
     gpu.close()
 
 
+STATE_FIELDS = ("content", "velocity", "new_velocity", "pressure", "divergence", "density", "particle_count",
+                "has_velocity", "volume_fractions", "new_volume_fractions")
+
+
+@pytest.mark.parametrize("scene,at_step", [("ball_drop_16", 25), ("ball_drop_64", 100)])
+def test_fp64_error_statement_mid_run(scene, at_step):
+    """The FP32 GPU step against an FP64 run of the same step at the 16^3 and 64^3 ball
+    drops: the GPU's whole state after `at_step` steps goes to the FP64 oracle and both
+    run one step (reported; the bound only guards against a broken comparison)."""
+    cfg = configs.SCENES[scene]()
+    gpu = FlipHandle(cfg)
+    gpu.steps(at_step)
+    pos, vel, frac = gpu.get_particles()
+    o64 = ob.Oracle(cfg, fp64=True, pos=pos, vel=vel, frac4=frac, init=False)
+    for f in STATE_FIELDS:
+        o64.set_field(f, gpu.get_field(f))
+    gpu.step()
+    o64.step()
+    for f in ("pressure", "new_velocity", "velocity"):
+        r = rel_l2(gpu.get_field(f), o64.get_field(f))
+        log(f"  [{scene} step {at_step + 1}] fp32-gpu vs fp64-oracle {f}: rel_l2={r:.3e}")
+        assert r < 1e-3
+    gp, gv, _ = gpu.get_particles()
+    op, ov, _ = o64.get_particles()
+    _, _, gh, pg = gpu.get_cells()
+    _, _, oh, po = o64.get_cells()
+    ig, io = np.empty_like(pg), np.empty_like(po)
+    ig[pg] = np.arange(pg.size)
+    io[po] = np.arange(po.size)
+    rp, rv = rel_l2(gp[ig], op[io]), rel_l2(gv[ig], ov[io])
+    moved = int((gh[ig] != oh[io]).sum())
+    log(f"  [{scene} step {at_step + 1}] particles: position rel_l2={rp:.3e}, velocity rel_l2={rv:.3e}, "
+        f"{moved} of {pg.size} in another cell")
+    assert rp < 1e-5 and rv < 1e-3
+    gpu.close()
+
+
 def test_solve_pressure_to_tolerance_16():
     cfg = configs.ball_drop_16()
     gpu = FlipHandle(cfg)
