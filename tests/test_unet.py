@@ -268,3 +268,24 @@ def test_device_dataset_and_adam_training():
     print(f"train loss {hist[:5].mean():.3e} -> {hist[-5:].mean():.3e}; held-out {before:.3e} -> {net.eval(8, 4):.3e}")
     assert np.isfinite(hist).all()
     assert hist[-10:].mean() < 0.7 * hist[:10].mean()
+
+
+@pytest.mark.gpu
+def test_training_leg_at_64_cubed_batch_8():
+    """A short leg of the cfg4 training as stated: pairs captured at 64^3 from the plain
+    solver on randomised scenes, Adam with lr 1e-3 and batch 8 on the device; the batch
+    loss falls and the held-out loss improves."""
+    from aquarius_amd import warmstart
+    from aquarius_amd.unet import UNet
+    net = UNet(seed=23, lr=1e-3)
+    info = warmstart.generate(net, 20, n=64, seed=5, per_scene=4, max_step=20, tol=1e-5, accept=1e-4,
+                              cap=30000, log=lambda *a: None, stall_sweeps=2000)
+    assert net.dataset_size == 20 and info["pairs"] == 20
+    before = net.eval(16, 4)
+    hist = net.train(60, 8, 16, seed=2)
+    after = net.eval(16, 4)
+    print(f"64^3 batch 8: train loss {hist[:5].mean():.3e} -> {hist[-5:].mean():.3e}; "
+          f"held-out {before:.3e} -> {after:.3e}")
+    assert np.isfinite(hist).all()
+    assert hist[-10:].mean() < 0.5 * hist[:10].mean()
+    assert after < before
