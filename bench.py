@@ -202,7 +202,7 @@ def tolerance_mode_steps(cfg, steps: int, cap: int, advance: int = 0):
     modes = [("plain", "tol_zero", None, 0), ("mgpcg", "mgpcg", None, 0)]
     if net is not None:
         modes += [("nn_guess_only", "tol_nn", net, 0), ("nn", "tol_nn", net, NN_REFRESH_TIMED)]
-    out = {"tol": 1e-4, "steps": steps, "first_step": advance + 2, "check_every": 50,
+    out = {"tol": 1e-4, "steps": steps, "first_step": advance + 2, "check_every": 100,
            "nn_refresh": NN_REFRESH_TIMED}
     for label, mode, nnet, refresh in modes:
         h = FlipHandle(cfg)
@@ -211,7 +211,7 @@ def tolerance_mode_steps(cfg, steps: int, cap: int, advance: int = 0):
         if mode == "mgpcg":
             attach_solver(h, mode, tol=1e-4, max_iters=500, check_every=1)
         else:
-            attach_solver(h, mode, tol=1e-4, max_iters=cap, check_every=50, net=nnet, nn_refresh=refresh)
+            attach_solver(h, mode, tol=1e-4, max_iters=cap, check_every=100, net=nnet, nn_refresh=refresh)
         h.steps(1)  # warm-up (first solve, lazy allocations)
         solver_stats(h, reset=True)
         t0 = time.perf_counter()
