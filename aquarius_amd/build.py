@@ -24,7 +24,9 @@ ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off keeps the reference kernels' rounding (no fused multiply-add),
-# and packed FP32 instructions are disabled for every HIP translation unit.
+# and packed FP32 instructions are disabled for every HIP translation unit. Kernel
+# arguments ahead of the first aggregate (up to 16 dwords) are preloaded into scalar
+# registers at wave launch (gfx950), so a kernel's first loads need no argument fetch.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
@@ -32,6 +34,7 @@ HIP_FLAGS = [
     "-fPIC",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}",
 ]

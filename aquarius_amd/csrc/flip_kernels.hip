@@ -1079,20 +1079,40 @@ __device__ __forceinline__ void jacobi_chunk(const Geom& G, const float* __restr
   for (int c = 0; c < kSweepCells; ++c)
     if (q[c].ok) pout[q[c].gi] = jacobi_value(b[c], v[c]);
 }
+// Values a sweep's prologue needs, in scalar registers at one point: all of their loads
+// (kernel arguments, the workgroup's table entry) then issue before one wait, instead of
+// one dependent round trip per group of uses. One asm statement, so that no register is
+// reused (and its load ordered after an earlier use) among them.
+#define AQF_SGPR_READY(G, e, ...)                                                               \
+  asm volatile("" ::"s"(e.x), "s"(e.y), "s"(e.z), "s"(e.w), "s"(G.X), "s"(G.Y), "s"(G.Z),    \
+               "s"(G.EY), "s"(G.EZ), "s"(G.sx), "s"(G.ncells_hash), "s"(G.divZ.m), "s"(G.divZ.s), \
+               "s"(G.divYZ.m), "s"(G.divYZ.s), __VA_ARGS__)
 template <bool PZERO>
-__global__ void k_jacobi(Geom G, const float* __restrict__ rhs, const float* __restrict__ pin,
-                         float* __restrict__ pout, const uint4* __restrict__ tab,
-                         const uint32_t* __restrict__ list, int full) {
-  for_each_chunk(tab, list, full != 0, [&](int ch) { jacobi_chunk<PZERO>(G, rhs, pin, pout, ch); });
+__global__ void k_jacobi(const uint4* __restrict__ tab, const uint32_t* __restrict__ list,
+                         const float* __restrict__ rhs, const float* __restrict__ pin,
+                         float* __restrict__ pout, int full, int nb, Geom G) {
+  const uint4 e = tab[full ? 0 : blockIdx.x];
+  const int nj = nb >= kXCD ? nb / kXCD : nb;
+  AQF_SGPR_READY(G, e, "s"(nj), "s"(nb));
+  if (full) {
+    const int per = nb / kXCD, rem = nb % kXCD, bid = blockIdx.x;
+    const int xcd = bid % kXCD, k = bid / kXCD;
+    jacobi_chunk<PZERO>(G, rhs, pin, pout, xcd * per + min(xcd, rem) + k);
+    return;
+  }
+  const int n = (int)e.x;
+  for (int k = 0; k < n; ++k)
+    jacobi_chunk<PZERO>(G, rhs, pin, pout, (int)(k == 0 ? e.z : k == 1 ? e.w : list[1 + e.y + k * nj]));
 }
 void launch_jacobi(const Geom& G, Grid gr, int pin, int full, int pzero, hipStream_t s) {
   const dim3 grid(full || pzero ? sweep_grid(G, true) : gr.listed[0]);
+  const int nb = (int)grid.x;
   if (pzero)
-    hipLaunchKernelGGL(k_jacobi<true>, grid, dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin], gr.p[pin ^ 1],
-                       gr.btab, gr.alist, 1);
+    hipLaunchKernelGGL(k_jacobi<true>, grid, dim3(kBlock), 0, s, gr.btab, gr.alist, gr.rhs, gr.p[pin],
+                       gr.p[pin ^ 1], 1, nb, G);
   else
-    hipLaunchKernelGGL(k_jacobi<false>, grid, dim3(kBlock), 0, s, G, gr.rhs, gr.p[pin], gr.p[pin ^ 1],
-                       gr.btab, gr.alist, full);
+    hipLaunchKernelGGL(k_jacobi<false>, grid, dim3(kBlock), 0, s, gr.btab, gr.alist, gr.rhs, gr.p[pin],
+                       gr.p[pin ^ 1], full, nb, G);
 }
 
 // diffusionJacobiImpl writes pressure = 0 on AIR cells (Fluid_3D_kernels.cu:529-532).
@@ -1401,20 +1421,54 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const PairedFields& 
 // skips its diffusion part: the results are those of running it. Sweep i also clears
 // the word sweep i + 1 will record into (word 0 is cleared by k_active_list, before
 // each step's first sweep). Word 3 counts the paired sweeps that computed diffusion.
+// Fields of a paired sweep past the ones the sweep's first loads need.
+struct PairedAux {
+  const uint8_t* content;
+  const uint32_t* active;
+  const float* fin[4];
+  float* fout[4];
+};
 template <int K, bool PZERO>
-__global__ void k_jacobi_diffusion(Geom G, PairedFields f, const uint4* __restrict__ tab,
-                                   const uint32_t* __restrict__ list, uint32_t* __restrict__ dchange,
-                                   int fullJ, float lambda, double inv_cc, int fullD, int sweep, int skip_ok) {
+__global__ void k_jacobi_diffusion(const uint4* __restrict__ tab, const uint32_t* __restrict__ list,
+                                   uint32_t* __restrict__ dchange, const float* __restrict__ rhs,
+                                   const float* __restrict__ pin, float* __restrict__ pout, int sweep,
+                                   int mode, int nb, PairedAux aux, float lambda, double inv_cc, Geom G) {
+  const bool fullJ = mode & 1, fullD = mode & 2, skip_ok = mode & 4;
   // Either sweep full: every chunk, each part where its own rule asks; otherwise the
   // active list, both parts. The previous sweep's change flag ((sweep + 2) % 3 is
-  // (sweep - 1) % 3 from sweep 1 on) and the table entry are loaded together.
+  // (sweep - 1) % 3 from sweep 1 on), the table entry and the geometry are loaded together.
   const bool full = fullJ || fullD;
   const uint32_t prev = dchange[(sweep + 2) % 3];
-  const uint4 e = tab[full ? 0u : blockIdx.x];  // unconditional, so it issues with the flag
+  const uint4 e = tab[full ? 0u : blockIdx.x];
+  const int nj = nb >= kXCD ? nb / kXCD : nb;
+  if (K == 1)
+    AQF_SGPR_READY(G, e, "s"(prev), "s"(aux.content), "s"(aux.active), "s"(lambda), "s"(inv_cc), "s"(nj),
+                   "s"(nb), "s"(aux.fin[0]), "s"(aux.fout[0]));
+  else if (K == 2)
+    AQF_SGPR_READY(G, e, "s"(prev), "s"(aux.content), "s"(aux.active), "s"(lambda), "s"(inv_cc), "s"(nj),
+                   "s"(nb), "s"(aux.fin[0]), "s"(aux.fout[0]), "s"(aux.fin[1]), "s"(aux.fout[1]));
+  else if (K == 3)
+    AQF_SGPR_READY(G, e, "s"(prev), "s"(aux.content), "s"(aux.active), "s"(lambda), "s"(inv_cc), "s"(nj),
+                   "s"(nb), "s"(aux.fin[0]), "s"(aux.fout[0]), "s"(aux.fin[1]), "s"(aux.fout[1]),
+                   "s"(aux.fin[2]), "s"(aux.fout[2]));
+  else
+    AQF_SGPR_READY(G, e, "s"(prev), "s"(aux.content), "s"(aux.active), "s"(lambda), "s"(inv_cc), "s"(nj),
+                   "s"(nb), "s"(aux.fin[0]), "s"(aux.fout[0]), "s"(aux.fin[1]), "s"(aux.fout[1]),
+                   "s"(aux.fin[2]), "s"(aux.fout[2]), "s"(aux.fin[3]), "s"(aux.fout[3]));
   const bool skipD = skip_ok && !fullD && sweep > 0 && prev == 0u;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     dchange[(sweep + 1) % 3] = 0u;
     if (!skipD) atomicAdd(&dchange[3], 1u);  // paired diffusion sweeps computed
+  }
+  PairedFields f;
+  f.rhs = rhs;
+  f.pin = pin;
+  f.pout = pout;
+  f.content = aux.content;
+  f.active = aux.active;
+  for (int k = 0; k < 4; ++k) {
+    f.fin[k] = aux.fin[k];
+    f.fout[k] = aux.fout[k];
   }
   bool changed = false;
   auto body = [&](int ch) {
@@ -1423,13 +1477,17 @@ __global__ void k_jacobi_diffusion(Geom G, PairedFields f, const uint4* __restri
     if (skipD) {  // the pressure sweep alone: no diffusion loads
       if (doJ) jacobi_chunk<PZERO>(G, f.rhs, f.pin, f.pout, ch);
     } else if (doJ || doD) {
-      paired_chunk<K, PZERO>(G, f, 0, 0, lambda, inv_cc, fullD != 0, doJ, doD, ch, changed);
+      paired_chunk<K, PZERO>(G, f, 0, 0, lambda, inv_cc, fullD, doJ, doD, ch, changed);
     }
   };
-  if (full)
-    body((int)xcd_block());
-  else
-    for_each_listed(e, list, body);
+  if (full) {
+    const int per = nb / kXCD, rem = nb % kXCD, bid = blockIdx.x;
+    const int xcd = bid % kXCD, k = bid / kXCD;
+    body(xcd * per + min(xcd, rem) + k);
+  } else {
+    const int n = (int)e.x;
+    for (int k = 0; k < n; ++k) body((int)(k == 0 ? e.z : k == 1 ? e.w : list[1 + e.y + k * nj]));
+  }
   // One plain store of 1 per wave that changed something: every writer stores the same
   // value, so the race is benign (a same-address atomic from every wave serialises: 39 us
   // per sweep in a mixing scene).
@@ -1440,22 +1498,21 @@ void launch_paired(const Geom& G, Grid gr, int pin, int fullJ, int fin, float la
                    int fullD, int pzero, int sweep, int skip_ok, hipStream_t s) {
   const dim3 grid(fullJ || fullD ? sweep_grid(G, true) : gr.listed[1]);
   const uint4* tab = gr.btab + gr.listed[0];  // the paired grid's entries
-  PairedFields f;
-  f.rhs = gr.rhs;
-  f.pin = gr.p[pin];
-  f.pout = gr.p[pin ^ 1];
-  f.content = gr.content;
-  f.active = gr.active;
+  PairedAux aux;
+  aux.content = gr.content;
+  aux.active = gr.active;
   for (int a = 0; a < 4; ++a) {
-    f.fin[a] = gr.nvf[fin][a];
-    f.fout[a] = gr.nvf[fin ^ 1][a];
+    aux.fin[a] = gr.nvf[fin][a];
+    aux.fout[a] = gr.nvf[fin ^ 1][a];
   }
+  const int mode = (fullJ ? 1 : 0) | (fullD ? 2 : 0) | (skip_ok ? 4 : 0);
+  const int nb = (int)grid.x;
   if (pzero)
-    hipLaunchKernelGGL((k_jacobi_diffusion<K, true>), grid, dim3(kBlock), 0, s, G, f, tab, gr.alist, gr.dchange,
-                       fullJ, lambda, inv_cc, fullD, sweep, skip_ok);
+    hipLaunchKernelGGL((k_jacobi_diffusion<K, true>), grid, dim3(kBlock), 0, s, tab, gr.alist, gr.dchange, gr.rhs,
+                       gr.p[pin], gr.p[pin ^ 1], sweep, mode, nb, aux, lambda, inv_cc, G);
   else
-    hipLaunchKernelGGL((k_jacobi_diffusion<K, false>), grid, dim3(kBlock), 0, s, G, f, tab, gr.alist, gr.dchange,
-                       fullJ, lambda, inv_cc, fullD, sweep, skip_ok);
+    hipLaunchKernelGGL((k_jacobi_diffusion<K, false>), grid, dim3(kBlock), 0, s, tab, gr.alist, gr.dchange, gr.rhs,
+                       gr.p[pin], gr.p[pin ^ 1], sweep, mode, nb, aux, lambda, inv_cc, G);
 }
 void launch_jacobi_diffusion(const Geom& G, Grid gr, int K, int pin, int fullJ, int fin, float lambda,
                              double inv_cc, int fullD, int pzero, int sweep, int skip_ok, hipStream_t s) {
