@@ -306,10 +306,29 @@ __global__ void k_scatter(const int* __restrict__ hash, const int* __restrict__ 
 
 // Only cells that received two or more runs of ranks (flagged by bin_particle) can be
 // out of order; every other thread reads its flag and returns, and every flag is
-// cleared for the next binning. Segments of up to kSegFast entries are checked from
-// registers loaded in one round trip (predicated loads); longer ones by a loop. An
-// unsorted segment is insertion-sorted in place.
+// cleared for the next binning. Segments of up to kSegFast entries are loaded into
+// registers in one round trip (predicated loads, INT_MAX past the end), sorted there by
+// a bitonic network (every index a compile-time constant, so the array stays in
+// registers) and written back once; the entries are distinct particle indices, so any
+// correct sort gives the same order. Longer segments are checked by a loop and, when
+// unsorted, insertion-sorted in place.
 constexpr int kSegFast = 16;
+__device__ __forceinline__ void sort16(int (&v)[kSegFast]) {
+#pragma unroll
+  for (int k = 2; k <= kSegFast; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < kSegFast; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const int lo = min(v[i], v[l]), hi = max(v[i], v[l]);
+          const bool up = (i & k) == 0;
+          v[i] = up ? lo : hi;
+          v[l] = up ? hi : lo;
+        }
+      }
+}
 __global__ void k_segsort(const int* __restrict__ offsets, int ncells, int* __restrict__ perm,
                           uint8_t* __restrict__ multi) {
   const int c = gtid();
@@ -323,9 +342,14 @@ __global__ void k_segsort(const int* __restrict__ offsets, int ncells, int* __re
     for (int k = 0; k < kSegFast; ++k) v[k] = s + k < e ? perm[s + k] : INT_MAX;
 #pragma unroll
     for (int k = 1; k < kSegFast; ++k) sorted &= s + k >= e || v[k - 1] < v[k];
-  } else {
-    for (int i = s + 1; i < e; ++i) sorted &= perm[i - 1] < perm[i];
+    if (sorted) return;
+    sort16(v);
+#pragma unroll
+    for (int k = 0; k < kSegFast; ++k)
+      if (s + k < e) perm[s + k] = v[k];
+    return;
   }
+  for (int i = s + 1; i < e; ++i) sorted &= perm[i - 1] < perm[i];
   if (sorted) return;
   for (int i = s + 1; i < e; ++i) {
     const int v = perm[i];
