@@ -131,6 +131,14 @@ def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps
     }
 
 
+def whole_step(kernels: dict, ms_per_step: float) -> dict:
+    """Algorithmic bytes of a whole step against its time: the step-level roofline."""
+    per_step = sum(k["bytes_per_launch"] * k["launches_per_step"] for k in kernels.values())
+    gbs = per_step / (ms_per_step * 1e-3) / 1e9
+    return {"bytes_per_step": round(per_step), "achieved_GBs": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 # Kernels of each stage, as rocprofv3 names them (template arguments stripped).
 STAGE_KERNELS = {
     "g2p": ["aqf::k_g2p_advect_hash"],
@@ -577,6 +585,9 @@ def main() -> int:
                    "particles": n, "grid": list(cfg.size), "pressure_iters": cfg.pressure_iters,
                    "diffusion_iters": cfg.diffusion_iters, "phases": K},
         "roofline": roofline,
+        # every stage's algorithmic bytes per step (launches x bytes per launch, from the
+        # profiled pass) over the timed step time
+        "whole_step": whole_step(kernels, ms_per_step),
         "kernels": kernels,
         "sweep_cells": {"all": cfg.size[0] * cfg.size[1] * cfg.size[2], "visited_when_listed": round(active_cells)},
         "paired_diffusion": {"sweeps_per_step": sweeps.get("pressure_diffusion", 0),
