@@ -495,7 +495,9 @@ template <int K>
 __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles P,
                                                    const int* __restrict__ off, Grid gr, int nv,
                                                    int hv, int nf, StepParams sp) {
-  __shared__ float4 sR[2][kP2GChunk];             // role records
+  // Role records; the first array padded by four records (64 bytes), so a cell's two
+  // lanes, which read slot c of each, hit banks 16 apart instead of the same banks.
+  __shared__ float4 sR[2][kP2GChunk + 4];
   const int b = xcd_block();
   const int xy = (int)fast_div((uint32_t)b, PG.divSeg);
   const int zs = b - xy * PG.nsegz;
