@@ -197,8 +197,8 @@ def load_warmstart_net():
     return net
 
 
-def tolerance_mode_steps(cfg, steps: int, cap: int, advance: int = 0):
-    """Steps/s with every step's pressure solved to a 1e-4 FP64 relative residual (or
+def tolerance_mode_steps(cfg, steps: int, cap: int, advance: int = 0, tol: float = 1e-4):
+    """Steps/s with every step's pressure solved to a `tol` FP64 relative residual (or
     until FP32 Jacobi stalls at its round-off floor), one fresh handle per solver:
     plain Jacobi from zero; the U-Net guess alone; the guess with a network correction
     every NN_REFRESH_TIMED sweeps (a correction that does not lower the residual is
@@ -210,16 +210,16 @@ def tolerance_mode_steps(cfg, steps: int, cap: int, advance: int = 0):
     modes = [("plain", "tol_zero", None, 0), ("mgpcg", "mgpcg", None, 0)]
     if net is not None:
         modes += [("nn_guess_only", "tol_nn", net, 0), ("nn", "tol_nn", net, NN_REFRESH_TIMED)]
-    out = {"tol": 1e-4, "steps": steps, "first_step": advance + 2, "check_every": 100,
+    out = {"tol": tol, "steps": steps, "first_step": advance + 2, "check_every": 100,
            "nn_refresh": NN_REFRESH_TIMED}
     for label, mode, nnet, refresh in modes:
         h = FlipHandle(cfg)
         if advance:
             h.steps(advance)
         if mode == "mgpcg":
-            attach_solver(h, mode, tol=1e-4, max_iters=500, check_every=1)
+            attach_solver(h, mode, tol=tol, max_iters=500, check_every=1)
         else:
-            attach_solver(h, mode, tol=1e-4, max_iters=cap, check_every=100, net=nnet, nn_refresh=refresh)
+            attach_solver(h, mode, tol=tol, max_iters=cap, check_every=100, net=nnet, nn_refresh=refresh)
         h.steps(1)  # warm-up (first solve, lazy allocations)
         solver_stats(h, reset=True)
         t0 = time.perf_counter()
@@ -237,6 +237,7 @@ def tolerance_mode_steps(cfg, steps: int, cap: int, advance: int = 0):
             row.update({"nn_corrections_per_step": st["nn_corrections"] / max(st["solves"], 1),
                         "nn_rejected": st["nn_rejected"]})
         out[label] = row
+        print(f"tolerance mode {label}: {row['steps_per_s']} steps/s", file=sys.stderr, flush=True)
     for label in ("nn_guess_only", "nn", "mgpcg"):
         if label in out:
             out[label]["speedup_vs_plain"] = round(out[label]["steps_per_s"] / out["plain"]["steps_per_s"], 3)
@@ -470,6 +471,9 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--timed-only", action="store_true",
                     help="warm-up, timed region and profiled pass only (for kernel traces)")
+    ap.add_argument("--scale-only", action="store_true",
+                    help="only the 256^3 ball drop in tolerance mode (1e-3; plain, guess, corrections, "
+                         "MG-PCG, 3 steps each) as its own JSON line")
     ap.add_argument("--unet-only", action="store_true",
                     help="time the U-Net inference on the workload's step-110 system only")
     args = ap.parse_args()
@@ -487,6 +491,12 @@ def main() -> int:
     cfg = configs.SCENES[args.scene]()
     if args.unet_only:
         print(json.dumps(unet_inference(cfg, args.solve_step)), flush=True)
+        return 0
+    if args.scale_only:
+        big = configs.ball_drop_256()
+        print(json.dumps({"workload": "ball_drop_256: 256x256x256 MAC grid, 8 particles per cell, "
+                                      "tolerance-mode steps 2-4 of fresh handles",
+                          "tolerance_mode": tolerance_mode_steps(big, 3, 200000, tol=1e-3)}), flush=True)
         return 0
     h = FlipHandle(cfg)
     n = h.particle_count
