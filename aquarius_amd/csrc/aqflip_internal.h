@@ -92,6 +92,9 @@ struct Grid {
   int listed[2];        // workgroups of a listed pressure sweep and of a listed paired sweep
   uint32_t* dchange;    // words 0-2: paired diffusion sweep i changed a bit -> word i % 3 != 0;
                         // word 3: paired diffusion sweeps computed (not skipped)
+  uint32_t* stepstamp;  // one word, advanced by the G2P pass that starts every step
+  uint32_t* rowstamp;   // per face row (x, y), x <= X, y <= Y: the stamp of the last step
+                        // whose P2G found FLUID in a cell row within 4 rows in x and in y
 };
 
 struct Bins {
@@ -167,7 +170,9 @@ void launch_fill_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 void launch_jacobi(const Geom& G, Grid gr, int pin, int full, int pzero, hipStream_t s);
 void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s);
-void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s);
+// near_rows: only the face rows stamped by this step's P2G (passes 2 and 3 of a step, see
+// k_extrapolate); otherwise every face.
+void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, bool near_rows, hipStream_t s);
 // full = 1: AIR cells copied to the output buffer; full = 0: AIR cells and blocks
 // without FLUID skipped -- valid once both buffers agree on AIR cells.
 void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc, int full,

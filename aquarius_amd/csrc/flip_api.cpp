@@ -327,6 +327,8 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
   AQ_TRY(h->allocT(&gr.btab, (size_t)sweep_table_entries(G)));
   set_listed_grids(G, gr, h->listed_grid);
   AQ_TRY(h->allocT(&gr.dchange, 4));
+  AQ_TRY(h->allocT(&gr.stepstamp, 1));
+  AQ_TRY(h->allocT(&gr.rowstamp, (size_t)(G.X + 1) * (G.Y + 1)));
   for (int a = 0; a < 4; ++a) {
     AQ_TRY(h->allocT(&gr.vf[a], nc));
     AQ_TRY(h->allocT(&gr.nvf[0][a], nc));
@@ -431,9 +433,11 @@ void fixed_sweeps(aq_flip* h) {
 
 void stage_pressure(aq_flip* h) { fixed_sweeps(h); }
 
-void stage_extrapolate(aq_flip* h) {
+// Inside a step (in_step) the second and third passes visit only the face rows this
+// step's P2G stamped (k_extrapolate); the stage on its own runs every pass in full.
+void stage_extrapolate(aq_flip* h, bool in_step = false) {
   for (int i = 0; i < 3; ++i) {  // maxDistance = 3 (Fluid_3D_common.cu:74)
-    launch_extrapolate(h->G, h->gr, h->nv, h->hv, h->stream);
+    launch_extrapolate(h->G, h->gr, h->nv, h->hv, in_step && i > 0, h->stream);
     h->nv ^= 1;
     h->hv ^= 1;
   }
@@ -837,7 +841,7 @@ aq_status enqueue_step(aq_flip* h) {
   launch_project(h->G, h->gr, h->nv, h->hv, h->pb, h->stream);
   h->end(AQ_STAGE_PROJECT);
   h->begin(AQ_STAGE_EXTRAPOLATE);
-  stage_extrapolate(h);
+  stage_extrapolate(h, true);
   h->end(AQ_STAGE_EXTRAPOLATE, 3);
   if (side) {
     if (h->cfg.pressure_iters == 0) launch_air_pressure_zero(h->G, h->gr, h->pb, h->stream);

@@ -696,6 +696,19 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
   }
   // role B's w-face results to its role-A partner
   const float tw = __shfl_xor(t1, 1, 64), sw = __shfl_xor(s1, 1, 64);
+  // A segment holding FLUID stamps the face rows within 4 rows in x and in y of its row:
+  // the only rows the step's last two extrapolation passes can change (k_extrapolate).
+  if (__ballot(active && role == 0 && ownHi > ownLo)) {
+    const uint32_t stamp = *gr.stepstamp;
+#pragma unroll
+    for (int k0 = 0; k0 < 81; k0 += kP2GWave) {
+      const int k = k0 + lane;
+      if (k < 81) {
+        const int fx = clampi(x + k / 9 - 4, 0, G.X), fy = clampi(y + k % 9 - 4, 0, G.Y);
+        gr.rowstamp[fx * (G.Y + 1) + fy] = stamp;
+      }
+    }
+  }
   if (!active || role) return;
   float tv0 = t1, tv1 = t2, tv2 = tw;
   uint8_t hvb = 0;
@@ -1205,24 +1218,69 @@ void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t 
 // neighbours that have velocity and whose velocity along the neighbour direction
 // passes the reference's epsilon test (epsilon = -1 + 1e-6). Faces outside the
 // launch domain are copied so both buffers stay complete.
-__global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
-  const int n = gtid();
-  if (n >= G.nfaces) return;
+//
+// Inside a step the last two passes visit only the face rows P2G stamped (near_rows).
+// That gives the same fields: projection rewrites every interior valid bit from FLUID
+// adjacency alone, so after it only faces within one row (in x and in y) of a cell row
+// holding FLUID have velocity, and each pass reaches one row further; rows beyond 4
+// never change. The step's first pass runs over every face, which leaves those rows
+// equal in both buffers, so the passes that skip them leave them as they would be.
+// The pointers and integers come ahead of the geometry (preloaded into scalar registers),
+// the grid size is an argument, and every scalar the prologue needs is named in one asm
+// statement: a wave's argument and geometry loads, and the step stamp, arrive in one
+// round trip, so a wave outside the near rows costs that and its row-stamp load.
+struct ExtrapFields {
+  const float* Ui;
+  const float* Vi;
+  const float* Wi;
+  float* Uo;
+  float* Vo;
+  float* Wo;
+  const uint8_t* Hi;
+  uint8_t* Ho;
+};
+// A face's row is its index divided by Z + 1 (x-major, z-fastest faces). With Z + 1 >= 64
+// a wave spans at most two rows, and a wave whose rows both lack this step's stamp leaves
+// after its first (scalar) round trip; `wide_rows` says the launcher found Z + 1 >= 64.
+__global__ void k_extrapolate(const uint32_t* __restrict__ rowstamp, const uint32_t* __restrict__ stepstamp,
+                              int near_rows, int nb, int nfaces, uint32_t z1m, uint32_t z1s, int wide_rows,
+                              ExtrapFields f, Geom G) {
+  const int per = nb / kXCD, rem = nb % kXCD, bid = blockIdx.x;
+  const int xcd = bid % kXCD;
+  const int n = (xcd * per + min(xcd, rem) + bid / kXCD) * kBlock + (int)threadIdx.x;
+  const FastDiv dz1{0u, z1m, z1s};
+  uint32_t stamp = 0u, s0 = 0u, s1 = 0u;
+  if (near_rows) {
+    stamp = *stepstamp;
+    if (wide_rows) {  // the wave's first and last rows, from scalar registers
+      const int w0 = __builtin_amdgcn_readfirstlane(n - (int)(threadIdx.x & 63));
+      s0 = rowstamp[fast_div((uint32_t)min(w0, nfaces - 1), dz1)];
+      s1 = rowstamp[fast_div((uint32_t)min(w0 + 63, nfaces - 1), dz1)];
+    }
+  }
+  asm volatile("" ::"s"(G.X), "s"(G.Y), "s"(G.Z), "s"(G.EY), "s"(G.EZ), "s"(G.sx), "s"(G.divYZ1.m),
+               "s"(G.divYZ1.s), "s"(G.divZ1.m), "s"(G.divZ1.s), "s"(stamp), "s"(s0), "s"(s1), "s"(f.Ui),
+               "s"(f.Vi), "s"(f.Wi), "s"(f.Uo), "s"(f.Vo), "s"(f.Wo), "s"(f.Hi), "s"(f.Ho));
+  if (near_rows && wide_rows && s0 != stamp && s1 != stamp) return;
+  if (n >= nfaces) return;
   int x, y, z;
   face_of(G, n, x, y, z);
+  if (near_rows && rowstamp[fast_div((uint32_t)n, dz1)] != stamp) return;
   const long long gi = G.g(x, y, z);
-  const float* __restrict__ Ui = gr.nvel[nvin][0];
-  const float* __restrict__ Vi = gr.nvel[nvin][1];
-  const float* __restrict__ Wi = gr.nvel[nvin][2];
-  float* __restrict__ Uo = gr.nvel[nvin ^ 1][0];
-  float* __restrict__ Vo = gr.nvel[nvin ^ 1][1];
-  float* __restrict__ Wo = gr.nvel[nvin ^ 1][2];
-  const uint8_t* __restrict__ Hi = gr.hasvel[hvin];
-  uint8_t* __restrict__ Ho = gr.hasvel[hvin ^ 1];
+  const float* __restrict__ Ui = f.Ui;
+  const float* __restrict__ Vi = f.Vi;
+  const float* __restrict__ Wi = f.Wi;
+  float* __restrict__ Uo = f.Uo;
+  float* __restrict__ Vo = f.Vo;
+  float* __restrict__ Wo = f.Wo;
+  const uint8_t* __restrict__ Hi = f.Hi;
+  uint8_t* __restrict__ Ho = f.Ho;
   if (x >= G.X || y >= G.Y || z >= G.Z) {
-    Uo[gi] = Ui[gi];
-    Vo[gi] = Vi[gi];
-    Wo[gi] = Wi[gi];
+    const float u = Ui[gi], v = Vi[gi], w = Wi[gi];
+    asm volatile("" ::"v"(u), "v"(v), "v"(w));  // three loads, then three stores
+    Uo[gi] = u;
+    Vo[gi] = v;
+    Wo[gi] = w;
     return;
   }
   const float eps = (float)(-1 + 1e-6);
@@ -1239,7 +1297,11 @@ __global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
   float nv[6][3];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const long long go = ok[k] ? gi + off[k] : gi;
+    // (the offset is hidden from the compiler, which would otherwise reuse the face's
+    // own loads for an out-of-range neighbour: wait for them, then load the others)
+    long long o = ok[k] ? off[k] : 0;
+    asm volatile("" : "+v"(o));
+    const long long go = gi + o;
     nh[k] = Hi[go];
 #pragma unroll
     for (int c = 0; c < 3; ++c) nv[k][c] = F[c][go];
@@ -1275,9 +1337,19 @@ __global__ void k_extrapolate(Geom G, Grid gr, int nvin, int hvin) {
   Wo[gi] = val[2];
   Ho[gi] = Hn;
 }
-void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, hipStream_t s) {
-  hipLaunchKernelGGL(k_extrapolate, dim3(nblocks(G.nfaces)), dim3(kBlock), 0, s, G, gr, nvin,
-                     hvin);
+void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, bool near_rows, hipStream_t s) {
+  ExtrapFields f;
+  f.Ui = gr.nvel[nvin][0];
+  f.Vi = gr.nvel[nvin][1];
+  f.Wi = gr.nvel[nvin][2];
+  f.Uo = gr.nvel[nvin ^ 1][0];
+  f.Vo = gr.nvel[nvin ^ 1][1];
+  f.Wo = gr.nvel[nvin ^ 1][2];
+  f.Hi = gr.hasvel[hvin];
+  f.Ho = gr.hasvel[hvin ^ 1];
+  const int nb = nblocks(G.nfaces);
+  hipLaunchKernelGGL(k_extrapolate, dim3(nb), dim3(kBlock), 0, s, gr.rowstamp, gr.stepstamp, near_rows ? 1 : 0,
+                     nb, G.nfaces, G.divZ1.m, G.divZ1.s, G.Z + 1 >= 64 ? 1 : 0, f, G);
 }
 
 // ------------------------------------------------------------------ diffusion
@@ -1700,6 +1772,7 @@ __global__ void k_g2p_advect_hash(Geom G, Particles P, long long Np, Grid gr, in
                                   int* __restrict__ rank, int* __restrict__ count,
                                   uint8_t* __restrict__ multi) {
   const int i = gtid();
+  if (i == 0) *gr.stepstamp += 1u;  // the step's stamp: no other thread of this pass reads it
   const bool valid = i < Np;
   int hh = 0;
   if (valid) {

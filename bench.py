@@ -78,8 +78,22 @@ def swept_cells(content, size) -> int:
     return int(min(chunks.sum() * SWEEP_SPAN, fluid.size))
 
 
+def near_row_faces(content, size) -> int:
+    """Faces the second and third extrapolation passes of a step visit: every face of
+    each face row (x <= X, y <= Y) within 4 rows in x and in y of a cell row holding a
+    FLUID cell (the rows P2G stamps)."""
+    import numpy as np
+    X, Y, Z = size
+    rows = (content[:X, :Y, :Z] == 1).any(2)
+    near = np.zeros((X + 1, Y + 1), bool)
+    for x, y in zip(*rows.nonzero()):
+        near[max(0, x - 4):x + 5, max(0, y - 4):y + 5] = True
+    return int(near.sum()) * (Z + 1)
+
+
 def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps: dict,
-                      diffusion_computed: float | None = None) -> dict:
+                      diffusion_computed: float | None = None,
+                      near_faces: float | None = None) -> dict:
     """Bytes each stage must move once per launch (every operand read or written
     exactly once), per the data layout of the library. `sweeps` holds the launches
     per step of the sweep stages. Sweeps count the cells a launch visits: all C
@@ -87,7 +101,9 @@ def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps
     rewritten in full after the FLUID set changes) and for the first diffusion sweep
     (it copies AIR cells across), the chunks holding FLUID otherwise. A paired sweep
     moves diffusion bytes only when it computed its diffusion part
-    (`diffusion_computed` per step, the first sweep of a step always)."""
+    (`diffusion_computed` per step, the first sweep of a step always). Of a step's three
+    extrapolation passes the first visits every face, the other two `near_faces` (all
+    of them when None)."""
     X, Y, Z = cfg.size
     C = X * Y * Z
     Np = n_particles
@@ -123,7 +139,8 @@ def algorithmic_bytes(cfg, n_particles: int, K: int, active_cells: float, sweeps
         # per sweep: p 4 r + rhs 4 r + mask 1 r + p 4 w per visited cell
         "pressure": 13 * per_launch(alone, min(alone, max(0, 2 - paired))),
         "project": C * (1 + 4 + 4 + 12 + 12 + 1),
-        "extrapolate": C * (12 + 1 + 12 + 1),  # per pass
+        # per pass: velocity and valid bits read and written per visited face
+        "extrapolate": (12 + 1 + 12 + 1) * (C if near_faces is None else (C + 2 * min(near_faces, C)) / 3),
         "diffusion": (1 + 8 * K) * per_launch(diff, 1 if diff and not paired else 0),
         # one pressure sweep and one diffusion sweep in the same launch
         "pressure_diffusion": (13 * per_launch(paired, min(2, paired))
@@ -534,7 +551,9 @@ def main() -> int:
     # one kernel launch per pass for the sweep stages.
     h.reset_timing()
     h.set_timing(True)
-    active0 = swept_cells(h.get_field("content"), cfg.size)
+    content0 = h.get_field("content")
+    active0 = swept_cells(content0, cfg.size)
+    near0 = near_row_faces(content0, cfg.size)
     computed0 = h.paired_diffusion_computed()
     tp0 = time.perf_counter()
     h.steps(args.steps)
@@ -543,11 +562,13 @@ def main() -> int:
     timing = h.timing()
     eager_ms_per_step = 1000.0 * (tp1 - tp0) / args.steps
     # cells in chunks with FLUID, averaged over the start and the end of the pass
-    active_cells = 0.5 * (active0 + swept_cells(h.get_field("content"), cfg.size))
+    content1 = h.get_field("content")
+    active_cells = 0.5 * (active0 + swept_cells(content1, cfg.size))
+    near_faces = 0.5 * (near0 + near_row_faces(content1, cfg.size))
     diffusion_computed = (h.paired_diffusion_computed() - computed0) / args.steps
 
     sweeps = {name: launches / args.steps for name, (ms, launches) in timing.items()}
-    nbytes = algorithmic_bytes(cfg, n, K, active_cells, sweeps, diffusion_computed)
+    nbytes = algorithmic_bytes(cfg, n, K, active_cells, sweeps, diffusion_computed, near_faces)
     kernels = {}
     for name, (ms, launches) in timing.items():
         if name not in nbytes or launches == 0:

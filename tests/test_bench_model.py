@@ -20,6 +20,19 @@ def test_algorithmic_bytes_closed_forms():
     # paired: 2 of 100 pressure sweeps full, diffusion computed once (the first, full)
     assert abs(b["pressure_diffusion"] - (13 * (2 * C + 98 * V) / 100 + (1 + 8 * K) * C / 100)) < 1e-6
     assert b["extrapolate"] == 26 * C
+    # second and third passes over the near rows only
+    b2 = bench.algorithmic_bytes(cfg, Np, K, V, sweeps, 1.0, near_faces=600_000)
+    assert abs(b2["extrapolate"] - 26 * (C + 2 * 600_000) / 3) < 1e-6
+
+
+def test_near_row_faces_box_of_nine_rows():
+    import numpy as np
+    X = Y = Z = 16
+    content = np.zeros((X + 1, Y + 1, Z + 1), np.int32)
+    content[8, 8, 3] = 1  # one FLUID cell: face rows 4..12 in x and in y
+    assert bench.near_row_faces(content, (X, Y, Z)) == 81 * (Z + 1)
+    content[0, 0, 0] = 1  # a corner: rows 0..4 (clipped), sharing row (4, 4) with the first box
+    assert bench.near_row_faces(content, (X, Y, Z)) == (81 + 25 - 1) * (Z + 1)
 
 
 def test_whole_step_sums_launches_times_bytes():
