@@ -29,6 +29,12 @@ __device__ __forceinline__ int xcd_block() {
   return xcd * per + min(xcd, rem) + k;
 }
 __device__ __forceinline__ int gtid() { return xcd_block() * kBlock + (int)threadIdx.x; }
+// xcd_block() for a launch of nb blocks passed as an argument (no hidden-argument load).
+__device__ __forceinline__ int xcd_block_n(int nb) {
+  const int per = nb / kXCD, rem = nb % kXCD, bid = blockIdx.x;
+  const int xcd = bid % kXCD;
+  return xcd * per + min(xcd, rem) + bid / kXCD;
+}
 
 static inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -765,18 +771,26 @@ void launch_p2g(const Geom& G, Particles P, long long Np, int K, Bins B, Grid gr
 // then fixBoundaryX/Y/Z (Fluid_3D_kernels.cu:44-98) with the reference's pair
 // coverage (Fluid_3D_common.cu:13-41). Runs over the whole (X+1)^3 face extent,
 // since the padding layer is both a gravity target and a boundary face.
-__global__ void k_gravity_boundary(Geom G, Grid gr, int nv, StepParams sp, int doGravity,
-                                   int doBoundary) {
-  const int n = gtid();
+// Every load of a face (its components and content, and the content of its -x, -y, -z
+// neighbours, read at the face itself where out of range) is issued before any is used.
+__global__ void k_gravity_boundary(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W,
+                                   uint8_t* __restrict__ content, int doGravity, int doBoundary, int nb,
+                                   StepParams sp, Geom G) {
+  asm volatile("" ::"s"(G.X), "s"(G.Y), "s"(G.Z), "s"(G.EY), "s"(G.EZ), "s"(G.sx), "s"(G.nfaces),
+               "s"(G.divZ1.m), "s"(G.divZ1.s), "s"(G.divYZ1.m), "s"(G.divYZ1.s), "s"(sp.dt), "s"(sp.g[0]),
+               "s"(sp.g[1]), "s"(sp.g[2]), "s"(sp.coverX), "s"(sp.coverY), "s"(sp.coverZ));
+  const int n = xcd_block_n(nb) * kBlock + (int)threadIdx.x;
   if (n >= G.nfaces) return;
   int x, y, z;
   face_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
-  float* __restrict__ U = gr.nvel[nv][0];
-  float* __restrict__ V = gr.nvel[nv][1];
-  float* __restrict__ W = gr.nvel[nv][2];
-  const uint8_t ct = gr.content[gi];
+  const uint8_t ct = content[gi];
   float u = U[gi], v = V[gi], w = W[gi];
+  long long ox = x >= 1 ? G.sx : 0, oy = y >= 1 ? (long long)G.EZ : 0, oz = z >= 1 ? 1 : 0;
+  asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));  // unconditional loads, one round trip
+  const uint8_t cxm = content[gi - ox], cym = content[gi - oy], czm = content[gi - oz];
+  asm volatile("" ::"v"(u), "v"(v), "v"(w), "v"((uint32_t)ct), "v"((uint32_t)cxm), "v"((uint32_t)cym),
+               "v"((uint32_t)czm));
   if (doGravity) {
     const float ax = sp.g[0] * sp.dt, ay = sp.g[1] * sp.dt, az = sp.g[2] * sp.dt;
     const bool inter = x < G.X && y < G.Y && z < G.Z;
@@ -786,20 +800,17 @@ __global__ void k_gravity_boundary(Geom G, Grid gr, int nv, StepParams sp, int d
       w += az;
     }
     if (ct == AIR) {
-      if (sp.g[0] != 0.f && x >= 1 && x - 1 < G.X && y < G.Y && z < G.Z &&
-          gr.content[gi - G.sx] == FLUID) {
+      if (sp.g[0] != 0.f && x >= 1 && x - 1 < G.X && y < G.Y && z < G.Z && cxm == FLUID) {
         u += ax;
         v += ay;
         w += az;
       }
-      if (sp.g[1] != 0.f && y >= 1 && x < G.X && y - 1 < G.Y && z < G.Z &&
-          gr.content[gi - G.EZ] == FLUID) {
+      if (sp.g[1] != 0.f && y >= 1 && x < G.X && y - 1 < G.Y && z < G.Z && cym == FLUID) {
         u += ax;
         v += ay;
         w += az;
       }
-      if (sp.g[2] != 0.f && z >= 1 && x < G.X && y < G.Y && z - 1 < G.Z &&
-          gr.content[gi - 1] == FLUID) {
+      if (sp.g[2] != 0.f && z >= 1 && x < G.X && y < G.Y && z - 1 < G.Z && czm == FLUID) {
         u += ax;
         v += ay;
         w += az;
@@ -821,12 +832,13 @@ __global__ void k_gravity_boundary(Geom G, Grid gr, int nv, StepParams sp, int d
   U[gi] = u;
   V[gi] = v;
   W[gi] = w;
-  if (nc != ct) gr.content[gi] = nc;
+  if (nc != ct) content[gi] = nc;
 }
 void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int doGravity,
                              int doBoundary, hipStream_t s) {
-  hipLaunchKernelGGL(k_gravity_boundary, dim3(nblocks(G.nfaces)), dim3(kBlock), 0, s, G, gr, nv,
-                     sp, doGravity, doBoundary);
+  const int nb = nblocks(G.nfaces);
+  hipLaunchKernelGGL(k_gravity_boundary, dim3(nb), dim3(kBlock), 0, s, gr.nvel[nv][0], gr.nvel[nv][1],
+                     gr.nvel[nv][2], gr.content, doGravity, doBoundary, nb, sp, G);
 }
 
 // ------------------------------------------------------------------ divergence
@@ -842,27 +854,49 @@ __device__ __forceinline__ void mark_active(const Grid& gr, bool fluid) {
   if (threadIdx.x == 0) gr.active[xcd_block()] = any ? 1 : 0;
 }
 
-__global__ void k_divergence(Geom G, Grid gr, int nv, const int* __restrict__ off) {
-  const int n = gtid();
+// Every load of a cell (content, the six face components, its two bin offsets, density)
+// is issued before the block-wide FLUID test: one memory round trip per cell.
+struct DivergenceFields {
+  const float* U;
+  const float* V;
+  const float* W;
+  const float* density;
+  float* div;
+  float* rhs;
+  uint32_t* active;
+};
+__global__ void k_divergence(const uint8_t* __restrict__ content, const int* __restrict__ off, int nb,
+                             DivergenceFields f, Geom G) {
+  asm volatile("" ::"s"(G.X), "s"(G.Y), "s"(G.Z), "s"(G.EY), "s"(G.EZ), "s"(G.sx), "s"(G.ncells_hash),
+               "s"(G.divZ.m), "s"(G.divZ.s), "s"(G.divYZ.m), "s"(G.divYZ.s), "s"(f.U), "s"(f.V), "s"(f.W),
+               "s"(f.density), "s"(f.div), "s"(f.rhs), "s"(f.active));
+  const int blk = xcd_block_n(nb);
+  const int n = blk * kBlock + (int)threadIdx.x;
   const bool in = n < G.ncells_hash;
-  int fx = 0, fy = 0, fz = 0;
-  if (in) cell_of(G, n, fx, fy, fz);
-  mark_active(gr, in && gr.content[G.g(fx, fy, fz)] == FLUID);
-  if (!in) return;
+  const int nn = in ? n : 0;  // past the end: cell 0's loads, unused
   int x, y, z;
-  cell_of(G, n, x, y, z);
+  cell_of(G, nn, x, y, z);
   const long long gi = G.g(x, y, z);
-  const float* __restrict__ U = gr.nvel[nv][0];
-  const float* __restrict__ V = gr.nvel[nv][1];
-  const float* __restrict__ W = gr.nvel[nv][2];
-  float div = V[gi + G.EZ] - V[gi] + U[gi + G.sx] - U[gi] + W[gi + 1] - W[gi];
-  const int cnt = off[n + 1] - off[n];
+  const float* __restrict__ U = f.U;
+  const float* __restrict__ V = f.V;
+  const float* __restrict__ W = f.W;
+  const uint8_t c = content[gi];
+  const float v1 = V[gi + G.EZ], v0 = V[gi], u1 = U[gi + G.sx], u0 = U[gi], w1 = W[gi + 1], w0 = W[gi];
+  const int o0 = off[nn], o1 = off[nn + 1];
+  const float dens = f.density[gi];
+  asm volatile("" ::"v"((uint32_t)c), "v"(v1), "v"(v0), "v"(u1), "v"(u0), "v"(w1), "v"(w0), "v"(o0), "v"(o1),
+               "v"(dens));
+  const int any = __syncthreads_or(in && c == FLUID ? 1 : 0);  // the block's active flag
+  if (threadIdx.x == 0) f.active[blk] = any ? 1 : 0;
+  if (!in) return;
+  float div = v1 - v0 + u1 - u0 + w1 - w0;
+  const int cnt = o1 - o0;
   const int pc = cnt > 0 ? cnt - 1 : 0;
   const double corr = (double)((float)pc - 8.0f) * 0.001;
   div = (float)((double)div - corr);
-  gr.div[gi] = div;
-  const bool air = gr.content[gi] == AIR;
-  gr.rhs[gi] = air ? __uint_as_float(kAirRhsBits) : -div * gr.density[gi];
+  f.div[gi] = div;
+  const bool air = c == AIR;
+  f.rhs[gi] = air ? __uint_as_float(kAirRhsBits) : -div * dens;
 }
 
 // Right-hand side from divergence, density and content (after a field upload).
@@ -883,8 +917,16 @@ void launch_refresh_rhs(const Geom& G, Grid gr, hipStream_t s) {
   launch_active_list(G, gr, s);
 }
 void launch_divergence(const Geom& G, Grid gr, int nv, Bins B, hipStream_t s) {
-  hipLaunchKernelGGL(k_divergence, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr, nv,
-                     B.offsets);
+  DivergenceFields f;
+  f.U = gr.nvel[nv][0];
+  f.V = gr.nvel[nv][1];
+  f.W = gr.nvel[nv][2];
+  f.density = gr.density;
+  f.div = gr.div;
+  f.rhs = gr.rhs;
+  f.active = gr.active;
+  const int nb = nblocks(G.ncells_hash);
+  hipLaunchKernelGGL(k_divergence, dim3(nb), dim3(kBlock), 0, s, gr.content, B.offsets, nb, f, G);
   launch_active_list(G, gr, s);
 }
 
@@ -1174,42 +1216,66 @@ void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s) {
 // updateVelocityWithPressureImpl (Fluid/Fluid_3D_kernels.cu:118-207): faces with a
 // FLUID side get u -= (p - p_other)/rho_used; hasVelocity is overwritten with
 // "face touches fluid".
-__global__ void k_project(Geom G, Grid gr, int nv, int hv, const float* __restrict__ p) {
-  const int n = gtid();
+// Every load of a cell (its content, pressure and density, those of its -x, -y, -z
+// neighbours, read at the cell itself where out of range, and its three face components)
+// is issued before any is used: one memory round trip per cell.
+struct ProjectFields {
+  float* F[3];
+  uint8_t* hv;
+};
+__global__ void k_project(const uint8_t* __restrict__ content, const float* __restrict__ p,
+                          const float* __restrict__ density, int nb, ProjectFields f, Geom G) {
+  asm volatile("" ::"s"(G.X), "s"(G.Y), "s"(G.Z), "s"(G.EY), "s"(G.EZ), "s"(G.sx), "s"(G.ncells_hash),
+               "s"(G.divZ.m), "s"(G.divZ.s), "s"(G.divYZ.m), "s"(G.divYZ.s), "s"(f.F[0]), "s"(f.F[1]),
+               "s"(f.F[2]), "s"(f.hv));
+  const int n = xcd_block_n(nb) * kBlock + (int)threadIdx.x;
   if (n >= G.ncells_hash) return;
   int x, y, z;
   cell_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
-  const uint8_t tc = gr.content[gi];
+  const uint8_t tc = content[gi];
   const float tp = p[gi];
-  const float td = gr.density[gi];
-  uint8_t hvb = 0;
+  const float td = density[gi];
   const int coord[3] = {x, y, z};
   const long long step[3] = {G.sx, (long long)G.EZ, 1};
+  uint32_t oc[3];
+  float op[3], od[3], fv[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    long long o = coord[d] > 0 ? step[d] : 0;
+    asm volatile("" : "+v"(o));  // unconditional loads, one round trip
+    const long long go = gi - o;
+    oc[d] = content[go];
+    op[d] = p[go];
+    od[d] = density[go];
+    fv[d] = f.F[d][gi];
+  }
+  asm volatile("" ::"v"((uint32_t)tc), "v"(tp), "v"(td));
+#pragma unroll
+  for (int d = 0; d < 3; ++d) asm volatile("" ::"v"(oc[d]), "v"(op[d]), "v"(od[d]), "v"(fv[d]));
+  uint8_t hvb = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (coord[d] > 0) {
-      const long long go = gi - step[d];
-      const uint8_t oc = gr.content[go];
-      const float op = p[go];
-      if (tc == FLUID || oc == FLUID) {
+      if (tc == FLUID || oc[d] == FLUID) {
         float used = td;
-        const float od = gr.density[go];
-        if (tc == FLUID && oc == FLUID)
-          used = (od + td) / 2.f;
-        else if (oc == FLUID)
-          used = od;
-        float* __restrict__ F = gr.nvel[nv][d];
-        F[gi] = F[gi] - (tp - op) / used;
+        if (tc == FLUID && oc[d] == FLUID)
+          used = (od[d] + td) / 2.f;
+        else if (oc[d] == FLUID)
+          used = od[d];
+        f.F[d][gi] = fv[d] - (tp - op[d]) / used;
         hvb |= (uint8_t)(1u << d);
       }
     }
   }
-  gr.hasvel[hv][gi] = hvb;
+  f.hv[gi] = hvb;
 }
 void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s) {
-  hipLaunchKernelGGL(k_project, dim3(nblocks(G.ncells_hash)), dim3(kBlock), 0, s, G, gr, nv, hv,
-                     gr.p[pb]);
+  ProjectFields f;
+  for (int d = 0; d < 3; ++d) f.F[d] = gr.nvel[nv][d];
+  f.hv = gr.hasvel[hv];
+  const int nb = nblocks(G.ncells_hash);
+  hipLaunchKernelGGL(k_project, dim3(nb), dim3(kBlock), 0, s, gr.content, gr.p[pb], gr.density, nb, f, G);
 }
 
 // ------------------------------------------------------------------ extrapolation
