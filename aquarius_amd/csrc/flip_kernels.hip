@@ -1080,25 +1080,6 @@ int sweep_grid(const Geom& G, bool full, int listed = kListedGrid) {
   if (full || nch < kXCD) return nch;
   return std::min(nch, listed) / kXCD * kXCD;
 }
-// A listed workgroup's chunks from its table entry e and, past the second, the list.
-template <typename F>
-__device__ __forceinline__ void for_each_listed(const uint4 e, const uint32_t* __restrict__ list, F&& body) {
-  const int n = (int)e.x;
-  const int nj = gridDim.x >= (unsigned)kXCD ? (int)gridDim.x / kXCD : (int)gridDim.x;
-  for (int k = 0; k < n; ++k) body((int)(k == 0 ? e.z : k == 1 ? e.w : list[1 + e.y + k * nj]));
-}
-// A listed workgroup reads its table entry (one scalar load, not dependent on the
-// list's length) and its chunks past the second from the list.
-template <typename F>
-__device__ __forceinline__ void for_each_chunk(const uint4* __restrict__ tab, const uint32_t* __restrict__ list,
-                                               bool full, F&& body) {
-  if (full) {
-    body((int)xcd_block());
-    return;
-  }
-  for_each_listed(tab[blockIdx.x], list, body);
-}
-
 // Cell c (0..kSweepCells-1) of this thread in chunk ch, its grid index and its six
 // clamp-to-edge neighbours in the reference's order x+1, x-1, y+1, y-1, z+1, z-1.
 // Threads past the last cell get cell 0 (in bounds) and ok = false.
