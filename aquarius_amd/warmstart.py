@@ -69,14 +69,55 @@ def converge(h: FlipHandle, tol: float, cap: int, check_every: int = 100, start=
                 return total, rel, False
 
 
+def add_residual_samples(net: UNet, h: FlipHandle, p_star: np.ndarray, sweeps) -> None:
+    """Samples of the residual equation the network's corrections solve. From zero, k
+    Jacobi sweeps give p_k; with clamp-to-edge neighbours the residual is
+    r = b - A p_k = 6 (J(p_k) - p_k) on FLUID cells (one more sweep gives J(p_k)), and
+    the target is the error left, e = p* - p_k. The handle's right-hand side is set to r
+    through its divergence field (rhs = -div * density) and its pressure to a multiple
+    of e for the sample, then both are restored. The guess and the corrections scale the
+    network's output by the energy-norm alpha, so only the error's shape matters; the
+    multiple gives the sample's target (pressure / RMS of the right-hand side) the RMS
+    the system's own sample has (the error left by a smooth residual is thousands of
+    times its RMS, which would otherwise swamp the loss)."""
+    X, Y, Z = h.field_shape("pressure")
+    fluid = h.get_field("content")[:X, :Y, :Z] == 1
+    div0 = h.get_field("divergence")
+    dens = h.get_field("density")
+    zero = np.zeros((X, Y, Z), np.float32)
+    safe = np.where(fluid, dens, 1.0).astype(np.float64)
+
+    def rms(a):
+        return float(np.sqrt(np.mean(np.square(a[fluid], dtype=np.float64))))
+
+    ratio = rms(p_star) / max(rms(-div0.astype(np.float64) * dens), 1e-30)  # the system's |p|/|b|
+    for k in sweeps:
+        h.set_field("pressure", zero)
+        h.solve_pressure(0.0, k, k)
+        pk = h.get_field("pressure").astype(np.float64)
+        h.solve_pressure(0.0, 1, 1)
+        r = np.where(fluid, 6.0 * (h.get_field("pressure") - pk), 0.0)
+        e = np.where(fluid, p_star - pk, 0.0)
+        c = ratio * rms(r) / max(rms(e), 1e-30)
+        h.set_field("divergence", np.where(fluid, -r / safe, div0).astype(np.float32))
+        h.set_field("pressure", (c * e).astype(np.float32))
+        net.add_sample(h)
+    h.set_field("divergence", div0)
+    h.set_field("pressure", p_star)
+
+
 def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int = 4,
              max_step: int = 60, tol: float = 1e-5, accept: float = 1e-4, cap: int = 60000,
-             log=print, solver: str = "jacobi", stall_sweeps: int = 0):
+             log=print, solver: str = "jacobi", stall_sweeps: int = 0, residual_sweeps=()):
     """Fill the network's device dataset with `pairs` converged systems. Within a
     scene each system is solved starting from the previous stop's converged pressure
     (the fixed point does not depend on the start, only the sweep count does). With
     stall_sweeps > 0 a Jacobi solve that stalls at its round-off floor above tol ends
-    there (and is kept if its residual is <= accept)."""
+    there (and is kept if its residual is <= accept). With residual_sweeps, each system
+    is followed by one residual-equation sample per entry (add_residual_samples), so
+    the dataset holds pairs * (1 + len(residual_sweeps)) samples, a system's together."""
+    per = 1 + len(residual_sweeps)
+    pairs *= per
     net.reserve(pairs, (n, n, n))
     rng = np.random.default_rng(seed)
     scene = 0
@@ -99,6 +140,8 @@ def generate(net: UNet, pairs: int, n: int = 64, seed: int = 0, per_scene: int =
             if rel <= accept and net.dataset_size < pairs:
                 net.add_sample(h)
                 prev = h.get_field("pressure") if solver != "mgpcg" else None
+                if residual_sweeps:
+                    add_residual_samples(net, h, h.get_field("pressure"), residual_sweeps)
         h.close()
         scene += 1
         if scene % 50 == 0:
@@ -201,6 +244,9 @@ def main(argv=None) -> int:
     t.add_argument("--per-scene", type=int, default=4, help="systems captured per random scene")
     t.add_argument("--stall-sweeps", type=int, default=0,
                    help="end a Jacobi solve after this many sweeps without a 1%% improvement (0: never)")
+    t.add_argument("--residual-sweeps", default="",
+                   help="comma-separated Jacobi sweep counts k: each system also gives a sample of the "
+                        "residual equation after k sweeps from zero (what corrections solve)")
     c = sub.add_parser("compare")
     c.add_argument("--weights", type=Path, default=WEIGHTS)
     c.add_argument("--n", type=int, default=64)
@@ -212,12 +258,15 @@ def main(argv=None) -> int:
     if a.cmd == "train":
         net = UNet(lr=a.lr, seed=a.seed + 1)
         total = a.pairs + a.held_out
-        print(f"[warmstart] generating {total} pairs at {a.n}^3", flush=True)
+        rs = tuple(int(k) for k in a.residual_sweeps.split(",") if k)
+        print(f"[warmstart] generating {total} systems at {a.n}^3"
+              + (f", each with residual samples after {list(rs)} sweeps" if rs else ""), flush=True)
         gen = generate(net, total, n=a.n, seed=a.seed, tol=a.tol, solver="mgpcg" if a.mgpcg else "jacobi",
-                       per_scene=a.per_scene, stall_sweeps=a.stall_sweeps)
+                       per_scene=a.per_scene, stall_sweeps=a.stall_sweeps, residual_sweeps=rs)
         print(f"[warmstart] data: {json.dumps(gen)}", flush=True)
-        train_n = a.pairs
-        l0 = net.eval(train_n, a.held_out)
+        per = 1 + len(rs)
+        train_n, held = a.pairs * per, a.held_out * per
+        l0 = net.eval(train_n, held)
         t0 = time.time()
         hist = []
         chunk = 250
@@ -228,7 +277,7 @@ def main(argv=None) -> int:
                 net.set_lr(a.lr_min + 0.5 * (a.lr - a.lr_min) * (1 + np.cos(np.pi * s / a.steps)))
             h = net.train(k, a.batch, train_n, seed=a.seed * 7919 + s)
             hist.append(float(h.mean()))
-            ho = net.eval(train_n, a.held_out)
+            ho = net.eval(train_n, held)
             if a.keep_best and ho < best[0]:
                 best = (ho, net.get_params(), s + k)
             print(f"[warmstart] step {s + k}: lr {net.cfg.lr:.2e} train loss {hist[-1]:.4e} "
@@ -236,7 +285,7 @@ def main(argv=None) -> int:
         if a.keep_best and best[1] is not None:
             net.set_params(best[1])
             print(f"[warmstart] kept the parameters of step {best[2]} (held-out {best[0]:.4e})", flush=True)
-        l1 = net.eval(train_n, a.held_out)
+        l1 = net.eval(train_n, held)
         a.out.parent.mkdir(parents=True, exist_ok=True)
         net.save(a.out)
         print(f"[warmstart] held-out loss {l0:.4e} -> {l1:.4e}; saved {a.out}", flush=True)

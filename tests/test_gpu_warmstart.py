@@ -96,3 +96,47 @@ def test_corrections_near_the_round_off_floor_are_undone_128():
     h.close()
     assert not ok and it == 30000 and nr == 1 and nc <= 40
     assert res <= 1.3 * plain_floor
+
+
+def test_residual_equation_samples_16():
+    """The residual samples the corrections train on: after k sweeps from zero,
+    6 (J(p_k) - p_k) on FLUID cells is the residual b - A p_k (its norm over ||b|| is the
+    library's FP64 relative residual at that point), and a sample holds r as its
+    right-hand side and p* - p_k as its target; the handle is left as it was."""
+    from aquarius_amd.unet import UNet
+    cfg = configs.ball_drop_16()
+    h = FlipHandle(cfg)
+    h.steps(3)
+    X, Y, Z = h.field_shape("pressure")
+    fluid = h.get_field("content")[:X, :Y, :Z] == 1
+    dens = h.get_field("density")
+    b = -h.get_field("divergence").astype(np.float64) * dens
+    warmstart.converge(h, 1e-6, 20000)
+    p_star = h.get_field("pressure")
+    div0 = h.get_field("divergence")
+    k = 40
+    h.set_field("pressure", np.zeros((X, Y, Z), np.float32))
+    _, rel_k, _ = h.solve_pressure(0.0, k, k)
+    pk = h.get_field("pressure").astype(np.float64)
+    h.solve_pressure(0.0, 1, 1)
+    r = np.where(fluid, 6.0 * (h.get_field("pressure") - pk), 0.0)
+    rel_host = np.linalg.norm(r[fluid]) / np.linalg.norm(b[fluid])
+    print(f"  relative residual after {k} sweeps: library {rel_k:.6e}, host {rel_host:.6e}")
+    assert abs(rel_host - rel_k) <= 1e-5 * rel_k
+    # the sample: right-hand side r, target p* - p_k; then the handle is restored
+    net = UNet()
+    net.reserve(2, (X, Y, Z))
+    h.set_field("pressure", p_star)
+    warmstart.add_residual_samples(net, h, p_star, (k,))
+    assert net.dataset_size == 1
+    np.testing.assert_array_equal(h.get_field("divergence"), div0)
+    np.testing.assert_array_equal(h.get_field("pressure"), p_star)
+    # re-create the sample's system by hand: its converged solution is p* - p_k
+    safe = np.where(fluid, dens, 1.0)
+    h.set_field("divergence", np.where(fluid, -r / safe, div0).astype(np.float32))
+    warmstart.converge(h, 1e-6, 20000)
+    e = h.get_field("pressure").astype(np.float64)
+    err = np.linalg.norm((e - (p_star - pk))[fluid]) / np.linalg.norm((p_star - pk)[fluid])
+    print(f"  solution of the residual equation against p* - p_k: {err:.3e}")
+    assert err < 1e-4  # observed 1.1e-6; a wrong sign or scale of r gives O(1)
+    h.close()
