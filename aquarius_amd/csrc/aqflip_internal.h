@@ -95,6 +95,10 @@ struct Grid {
   uint32_t* stepstamp;  // one word, advanced by the G2P pass that starts every step
   uint32_t* rowstamp;   // per face row (x, y), x <= X, y <= Y: the stamp of the last step
                         // whose P2G found FLUID in a cell row within 4 rows in x and in y
+  float2* nvo[3];       // per face: (newVelocity, velocity) interleaved, for the G2P sample
+                        // of the next step (one 16-byte load per corner pair, both fields);
+                        // written by a step's last extrapolation pass on the rows it visits,
+                        // or in full by launch_build_nvo
 };
 
 struct Bins {
@@ -172,7 +176,10 @@ void launch_air_pressure_zero(const Geom& G, Grid gr, int pb, hipStream_t s);
 void launch_project(const Geom& G, Grid gr, int nv, int hv, int pb, hipStream_t s);
 // near_rows: only the face rows stamped by this step's P2G (passes 2 and 3 of a step, see
 // k_extrapolate); otherwise every face.
-void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, bool near_rows, hipStream_t s);
+void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, bool near_rows, hipStream_t s,
+                        bool write_nvo = false);
+// gr.nvo from newVelocity buffer nv and velocity, every face.
+void launch_build_nvo(const Geom& G, Grid gr, int nv, hipStream_t s);
 // full = 1: AIR cells copied to the output buffer; full = 0: AIR cells and blocks
 // without FLUID skipped -- valid once both buffers agree on AIR cells.
 void launch_diffusion(const Geom& G, Grid gr, int K, int fin, float lambda, double inv_cc, int full,

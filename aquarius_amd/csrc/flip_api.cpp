@@ -122,6 +122,7 @@ struct aq_flip {
   };
   std::vector<StepGraph> graphs;
   bool use_graphs = true;
+  bool nvo_valid = false;  // gr.nvo matches newVelocity and velocity (see graph_step)
   // Tolerance solves: k Jacobi sweeps plus the FP64 residual kernels between two
   // host checks, captured once per (pressure buffer, AIR-clean state, k).
   struct SweepGraph {
@@ -313,6 +314,7 @@ aq_status setup(aq_flip* h, const aq_flip_config* cfg, long long Np) {
     AQ_TRY(h->allocT(&gr.nvel[0][a], nc));
     AQ_TRY(h->allocT(&gr.nvel[1][a], nc));
   }
+  for (int a = 0; a < 3; ++a) AQ_TRY(h->allocT(&gr.nvo[a], nc));
   AQ_TRY(h->allocT(&gr.hasvel[0], nc));
   AQ_TRY(h->allocT(&gr.hasvel[1], nc));
   AQ_TRY(h->allocT(&gr.p[0], nc));
@@ -435,9 +437,11 @@ void stage_pressure(aq_flip* h) { fixed_sweeps(h); }
 
 // Inside a step (in_step) the second and third passes visit only the face rows this
 // step's P2G stamped (k_extrapolate); the stage on its own runs every pass in full.
+// Inside a step the last pass also writes the interleaved (newVelocity, velocity) copy the
+// next step's G2P samples (gr.nvo), on the rows it visits.
 void stage_extrapolate(aq_flip* h, bool in_step = false) {
   for (int i = 0; i < 3; ++i) {  // maxDistance = 3 (Fluid_3D_common.cu:74)
-    launch_extrapolate(h->G, h->gr, h->nv, h->hv, in_step && i > 0, h->stream);
+    launch_extrapolate(h->G, h->gr, h->nv, h->hv, in_step && i > 0, h->stream, in_step && i == 2);
     h->nv ^= 1;
     h->hv ^= 1;
   }
@@ -492,6 +496,7 @@ int stage_diffusion(aq_flip* h, bool inStep) {
 aq_status run_stage(aq_flip* h, int st) {
   if (st == AQ_STAGE_PRESSURE_DIFFUSION || st == AQ_STAGE_P2G)
     return fail(AQ_EINVAL, "timing bucket: runs only inside a step");
+  h->nvo_valid = false;
   if (st < 0 || st >= AQ_STAGE_COUNT) return fail(AQ_EINVAL, "unknown stage");
   int64_t passes = 1;
   h->begin(st);
@@ -875,7 +880,16 @@ void set_state(aq_flip* h, const int k[7]) {
 
 // One step through a captured graph (eager launches when timing is on, so the
 // stage events sit on the stream between kernels).
+// A step's G2P reads new and old velocity from gr.nvo, which the previous step's last
+// extrapolation pass wrote on every row a particle's first sample can touch (the rows
+// within 4 of a cell row holding FLUID; a particle's corners lie within one row of its
+// FLUID cell). Anything else that changes the velocities, or a fresh handle, leaves it
+// stale, and the copy is then rebuilt in full first.
 aq_status graph_step(aq_flip* h) {
+  if (!h->nvo_valid) {
+    launch_build_nvo(h->G, h->gr, h->nv, h->stream);
+    h->nvo_valid = true;
+  }
   if (h->timing || !h->use_graphs || h->solver.mode != AQ_SOLVE_FIXED) return enqueue_step(h);
   int key[7];
   state_of(h, key);
@@ -1389,6 +1403,7 @@ aq_status aq_flip_get_field(aq_flip* h, int32_t field, void* dst, size_t bytes) 
 
 aq_status aq_flip_set_field(aq_flip* h, int32_t field, const void* src, size_t bytes) {
   if (!h || !src) return fail(AQ_EINVAL, "null argument");
+  h->nvo_valid = false;
   int w;
   bool isInt;
   const size_t n = field_elems(h, field, &w, &isInt);

@@ -1285,13 +1285,15 @@ struct ExtrapFields {
   float* Wo;
   const uint8_t* Hi;
   uint8_t* Ho;
+  const float* old[3];  // velocity, read with write_nvo
+  float2* nvo[3];       // with write_nvo: (new value, velocity) of every visited face
 };
 // A face's row is its index divided by Z + 1 (x-major, z-fastest faces). With Z + 1 >= 64
 // a wave spans at most two rows, and a wave whose rows both lack this step's stamp leaves
 // after its first (scalar) round trip; `wide_rows` says the launcher found Z + 1 >= 64.
 __global__ void k_extrapolate(const uint32_t* __restrict__ rowstamp, const uint32_t* __restrict__ stepstamp,
                               int near_rows, int nb, int nfaces, uint32_t z1m, uint32_t z1s, int wide_rows,
-                              ExtrapFields f, Geom G) {
+                              int write_nvo, ExtrapFields f, Geom G) {
   const int per = nb / kXCD, rem = nb % kXCD, bid = blockIdx.x;
   const int xcd = bid % kXCD;
   const int n = (xcd * per + min(xcd, rem) + bid / kXCD) * kBlock + (int)threadIdx.x;
@@ -1322,12 +1324,22 @@ __global__ void k_extrapolate(const uint32_t* __restrict__ rowstamp, const uint3
   float* __restrict__ Wo = f.Wo;
   const uint8_t* __restrict__ Hi = f.Hi;
   uint8_t* __restrict__ Ho = f.Ho;
+  float ov[3] = {0.f, 0.f, 0.f};
+  if (write_nvo) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ov[c] = f.old[c][gi];
+  }
   if (x >= G.X || y >= G.Y || z >= G.Z) {
     const float u = Ui[gi], v = Vi[gi], w = Wi[gi];
-    asm volatile("" ::"v"(u), "v"(v), "v"(w));  // three loads, then three stores
+    asm volatile("" ::"v"(u), "v"(v), "v"(w), "v"(ov[0]), "v"(ov[1]), "v"(ov[2]));  // loads, then stores
     Uo[gi] = u;
     Vo[gi] = v;
     Wo[gi] = w;
+    if (write_nvo) {
+      f.nvo[0][gi] = make_float2(u, ov[0]);
+      f.nvo[1][gi] = make_float2(v, ov[1]);
+      f.nvo[2][gi] = make_float2(w, ov[2]);
+    }
     return;
   }
   const float eps = (float)(-1 + 1e-6);
@@ -1353,7 +1365,7 @@ __global__ void k_extrapolate(const uint32_t* __restrict__ rowstamp, const uint3
 #pragma unroll
     for (int c = 0; c < 3; ++c) nv[k][c] = F[c][go];
   }
-  asm volatile("" ::"v"(val[0]), "v"(val[1]), "v"(val[2]));
+  asm volatile("" ::"v"(val[0]), "v"(val[1]), "v"(val[2]), "v"(ov[0]), "v"(ov[1]), "v"(ov[2]));
 #pragma unroll
   for (int k = 0; k < 6; ++k) asm volatile("" ::"v"(nh[k]), "v"(nv[k][0]), "v"(nv[k][1]), "v"(nv[k][2]));
   uint8_t Hn = H;
@@ -1383,8 +1395,31 @@ __global__ void k_extrapolate(const uint32_t* __restrict__ rowstamp, const uint3
   Vo[gi] = val[1];
   Wo[gi] = val[2];
   Ho[gi] = Hn;
+  if (write_nvo) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f.nvo[c][gi] = make_float2(val[c], ov[c]);
+  }
 }
-void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, bool near_rows, hipStream_t s) {
+// gr.nvo in full: (newVelocity buffer nv, velocity) of every face.
+__global__ void k_build_nvo(Geom G, const float* __restrict__ n0, const float* __restrict__ n1,
+                            const float* __restrict__ n2, const float* __restrict__ o0,
+                            const float* __restrict__ o1, const float* __restrict__ o2, float2* __restrict__ d0,
+                            float2* __restrict__ d1, float2* __restrict__ d2) {
+  const int n = gtid();
+  if (n >= G.nfaces) return;
+  int x, y, z;
+  face_of(G, n, x, y, z);
+  const long long gi = G.g(x, y, z);
+  d0[gi] = make_float2(n0[gi], o0[gi]);
+  d1[gi] = make_float2(n1[gi], o1[gi]);
+  d2[gi] = make_float2(n2[gi], o2[gi]);
+}
+void launch_build_nvo(const Geom& G, Grid gr, int nv, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_nvo, dim3(nblocks(G.nfaces)), dim3(kBlock), 0, s, G, gr.nvel[nv][0], gr.nvel[nv][1],
+                     gr.nvel[nv][2], gr.vel[0], gr.vel[1], gr.vel[2], gr.nvo[0], gr.nvo[1], gr.nvo[2]);
+}
+void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, bool near_rows, hipStream_t s,
+                        bool write_nvo) {
   ExtrapFields f;
   f.Ui = gr.nvel[nvin][0];
   f.Vi = gr.nvel[nvin][1];
@@ -1394,9 +1429,13 @@ void launch_extrapolate(const Geom& G, Grid gr, int nvin, int hvin, bool near_ro
   f.Wo = gr.nvel[nvin ^ 1][2];
   f.Hi = gr.hasvel[hvin];
   f.Ho = gr.hasvel[hvin ^ 1];
+  for (int c = 0; c < 3; ++c) {
+    f.old[c] = gr.vel[c];
+    f.nvo[c] = gr.nvo[c];
+  }
   const int nb = nblocks(G.nfaces);
   hipLaunchKernelGGL(k_extrapolate, dim3(nb), dim3(kBlock), 0, s, gr.rowstamp, gr.stepstamp, near_rows ? 1 : 0,
-                     nb, G.nfaces, G.divZ1.m, G.divZ1.s, G.Z + 1 >= 64 ? 1 : 0, f, G);
+                     nb, G.nfaces, G.divZ1.m, G.divZ1.s, G.Z + 1 >= 64 ? 1 : 0, write_nvo ? 1 : 0, f, G);
 }
 
 // ------------------------------------------------------------------ diffusion
@@ -1723,12 +1762,54 @@ __device__ __forceinline__ void store_rec(const Particles& P, int i, const PRec&
 
 // transferToParticlesImpl (Fluid/Fluid_3D_FLIP.cu:189-227). Returns newV at the
 // particle, which is also RK3's first stage.
+// interp on the interleaved (new, old) copy: the same corners, lerps and order for both
+// fields, each corner pair of both in one 16-byte load.
+__device__ __forceinline__ void interp2(const Geom& G, const float2* __restrict__ F, float x, float y, float z,
+                                        float& nout, float& oout) {
+  x = fmaxf(fminf(x, (float)G.X - 1.f), 0.f);
+  y = fmaxf(fminf(y, (float)G.Y - 1.f), 0.f);
+  z = fmaxf(fminf(z, (float)G.Z - 1.f), 0.f);
+  const int i = (int)floorf(x), j = (int)floorf(y), k = (int)floorf(z);
+  const float tx = x - (float)i, ty = y - (float)j, tz = z - (float)k;
+  const long long b = G.g(i, j, k);
+  const long long sx = G.sx, sy = G.EZ;
+  struct alignas(8) Pair2 {  // faces k and k + 1 of a z row: (new, old) each
+    float n0, o0, n1, o1;
+  };
+  auto pair = [&](long long o) {
+    Pair2 v;
+    __builtin_memcpy(&v, F + o, sizeof v);
+    return v;
+  };
+  const Pair2 a = pair(b), c = pair(b + sy), d = pair(b + sx), e = pair(b + sx + sy);
+  const float n00 = lerpf_(a.n0, a.n1, tz), n01 = lerpf_(c.n0, c.n1, tz);
+  const float n10 = lerpf_(d.n0, d.n1, tz), n11 = lerpf_(e.n0, e.n1, tz);
+  const float o00 = lerpf_(a.o0, a.o1, tz), o01 = lerpf_(c.o0, c.o1, tz);
+  const float o10 = lerpf_(d.o0, d.o1, tz), o11 = lerpf_(e.o0, e.o1, tz);
+  nout = lerpf_(lerpf_(n00, n01, ty), lerpf_(n10, n11, ty), tx);
+  oout = lerpf_(lerpf_(o00, o01, ty), lerpf_(o10, o11, ty), tx);
+}
+__device__ __forceinline__ void point_velocity2(const Geom& G, const float2* const NO[3], float px, float py,
+                                                float pz, float nvp[3], float ovp[3]) {
+  const float x = div_exact(px, G.inv_h), y = div_exact(py, G.inv_h), z = div_exact(pz, G.inv_h);
+  interp2(G, NO[0], x, y - 0.5f, z - 0.5f, nvp[0], ovp[0]);
+  interp2(G, NO[1], x - 0.5f, y, z - 0.5f, nvp[1], ovp[1]);
+  interp2(G, NO[2], x - 0.5f, y - 0.5f, z, nvp[2], ovp[2]);
+}
+
+// interleaved: new and old velocity from gr.nvo (valid at the start of a step);
+// otherwise from newVelocity buffer nv and velocity.
 template <int K>
 __device__ __forceinline__ void g2p_one(const Geom& G, PRec& r, const Grid& gr, int nv, int nf,
-                                        float flip, float nvOut[3]) {
+                                        float flip, float nvOut[3], bool interleaved = false) {
   float nvp[3], ovp[3];
-  point_velocity(G, gr.nvel[nv][0], gr.nvel[nv][1], gr.nvel[nv][2], r.p[0], r.p[1], r.p[2], nvp);
-  point_velocity(G, gr.vel[0], gr.vel[1], gr.vel[2], r.p[0], r.p[1], r.p[2], ovp);
+  if (interleaved) {
+    const float2* const NO[3] = {gr.nvo[0], gr.nvo[1], gr.nvo[2]};
+    point_velocity2(G, NO, r.p[0], r.p[1], r.p[2], nvp, ovp);
+  } else {
+    point_velocity(G, gr.nvel[nv][0], gr.nvel[nv][1], gr.nvel[nv][2], r.p[0], r.p[1], r.p[2], nvp);
+    point_velocity(G, gr.vel[0], gr.vel[1], gr.vel[2], r.p[0], r.p[1], r.p[2], ovp);
+  }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const float change = nvp[c] - ovp[c];
@@ -1826,7 +1907,7 @@ __global__ void k_g2p_advect_hash(Geom G, Particles P, long long Np, Grid gr, in
     PRec r;
     load_rec<K>(P, i, r);
     float u1[3];
-    g2p_one<K>(G, r, gr, nv, nf, flip, u1);
+    g2p_one<K>(G, r, gr, nv, nf, flip, u1, true);
     advect_one(G, r, gr, nv, dt, u1);
     store_rec<K>(P, i, r);
     hh = cell_hash(G, r.p[0], r.p[1], r.p[2]);
