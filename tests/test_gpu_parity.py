@@ -436,3 +436,40 @@ def test_stage_timing_counts_and_matches_graph(monkeypatch):
     t = h.timing()
     h.close()
     assert t["pressure_diffusion"][1] == 0 and t["pressure"][1] == 40 and t["diffusion"][1] == 10
+
+
+def test_full_steps_match_oracle_non_cubic_64_rows():
+    """Whole steps on a 20 x 24 x 64 grid (X != Y, and z rows of 65 faces, so a wave of
+    the last two extrapolation passes spans at most two face rows and tests them both
+    with scalar loads) against the oracle: the near-row passes, their row stamps and the
+    interleaved velocity copy the next step's G2P reads."""
+    cfg = configs.make_config(0, dt=0.01, pressure_iters=30, diffusion_iters=5, extent=(20, 24, 64),
+                              sphere_phase=1, phase_count=2, radius=0.25, center=(0.45, 0.65, 0.55), seed=11)
+    gpu, orc = pair(cfg)
+    log("[full steps 20x24x64] 6 steps")
+    gpu.steps(6)
+    orc.step(6)
+    check_particles(gpu, orc, "6 steps ")
+    check_cells(gpu, orc, "6 steps ")
+    for f in ("pressure", "new_velocity", "velocity", "has_velocity", "content", "volume_fractions"):
+        check_field(f, gpu, orc, "6 steps ")
+    gpu.close()
+
+
+def test_steps_after_a_field_upload_match_an_uninterrupted_run():
+    """Uploading a velocity field (here unchanged) marks the interleaved (newVelocity,
+    velocity) copy stale; the next step rebuilds it in full and must continue exactly as
+    a run without the upload."""
+    cfg = configs.ball_drop_64()
+    a, b = FlipHandle(cfg), FlipHandle(cfg)
+    a.steps(8)
+    b.steps(7)
+    b.set_field("new_velocity", b.get_field("new_velocity"))
+    b.set_field("velocity", b.get_field("velocity"))
+    b.steps(1)
+    for f in ("pressure", "new_velocity", "velocity", "has_velocity", "content"):
+        np.testing.assert_array_equal(a.get_field(f), b.get_field(f), err_msg=f)
+    for x, y in zip(a.get_particles(), b.get_particles()):
+        np.testing.assert_array_equal(x, y)
+    a.close()
+    b.close()
