@@ -504,6 +504,9 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
   // Role records; the first array padded by four records (64 bytes), so a cell's two
   // lanes, which read slot c of each, hit banks 16 apart instead of the same banks.
   __shared__ float4 sR[2][kP2GChunk + 4];
+  // Kept records before each staged position (0..kP2GChunk): a lane's window maps onto
+  // the kept records with two lookups.
+  __shared__ uint16_t sPre[kP2GChunk + 1];
   const int b = xcd_block();
   const int xy = (int)fast_div((uint32_t)b, PG.divSeg);
   const int zs = b - xy * PG.nsegz;
@@ -591,14 +594,10 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
       ncs = nsl < 9 ? win_lo(nsl) : 0;
     }
     __syncthreads();
-    uint64_t kept[kP2GChunk / kP2GWave];
     int nkept = 0;
 #pragma unroll
     for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
-      if (bi * kP2GWave >= nst) {
-        kept[bi] = 0;
-        continue;
-      }
+      if (bi * kP2GWave >= nst) continue;
       const int k = bi * kP2GWave + lane;
       bool keep = false;
       const float4 A = LA[bi];
@@ -613,11 +612,11 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
         keep = centre || pu != 0.f || pv != 0.f || pw != 0.f;
       }
       const uint64_t m = __ballot(keep);
-      kept[bi] = m;
+      const int q = nkept + __popcll(m & ((1ull << lane) - 1));  // kept before position k
+      sPre[k] = (uint16_t)q;
       if (keep) {
         // Narrow stores of the loaded registers as they are: a 16-byte store of values
         // from two loads would copy them first, and a copy waits for the load.
-        const int q = nkept + __popcll(m & ((1ull << lane) - 1));
         float* r0 = reinterpret_cast<float*>(&sR[0][q]);
         float* r1 = reinterpret_cast<float*>(&sR[1][q]);
         *reinterpret_cast<float2*>(r0) = make_float2(pu, pv);
@@ -634,20 +633,11 @@ __global__ __launch_bounds__(kP2GWave) void k_p2g(Geom G, P2GGeom PG, Particles 
       }
       nkept += __popcll(m);
     }
+    if (lane == 0) sPre[nst] = (uint16_t)nkept;  // nst a multiple of 64: no lane wrote it
     if (!centre && nsl < 9) load_item(nsl, ncs);
     __syncthreads();
-    auto kept_before = [&](int o) {
-      int c = 0;
-#pragma unroll
-      for (int bi = 0; bi < kP2GChunk / kP2GWave; ++bi) {
-        const int b0 = bi * kP2GWave;
-        if (o >= b0 + kP2GWave) c += __popcll(kept[bi]);
-        else if (o > b0) c += __popcll(kept[bi] & ((1ull << (o - b0)) - 1));
-      }
-      return c;
-    };
-    const int c0 = kept_before(clampi(lo - cs, 0, nst));
-    const int c1 = kept_before(clampi(hi - cs, 0, nst));
+    const int c0 = sPre[clampi(lo - cs, 0, nst)];
+    const int c1 = sPre[clampi(hi - cs, 0, nst)];
     auto accumulate = [&](int c, bool ownc) {
       const float4 R = myR[c];
       const float vy = sR[1][c].y;
