@@ -78,6 +78,17 @@ def test_gpu_random_records(stride):
 
 
 @pytest.mark.gpu
+def test_gpu_segments_of_17_to_32_records():
+    # about 24 records per cell in random order (Poisson counts, mostly 17-32): each cell
+    # receives runs from many waves, so its segment is sorted by the 32-entry network
+    rng = np.random.default_rng(24)
+    pos = rng.uniform(0, 16, (16 ** 3 * 24, 3)).astype(np.float32)
+    counts = np.bincount((pos.astype(np.int64) * [256, 16, 1]).sum(1), minlength=16 ** 3)
+    assert ((counts > 16) & (counts <= 32)).sum() > 3000
+    gpu_case((16, 16, 16), 1.0, pos)
+
+
+@pytest.mark.gpu
 def test_gpu_edge_cases():
     gpu_case((3, 4, 5), 1.0, np.zeros((0, 3), np.float32))                      # empty
     gpu_case((3, 4, 5), 1.0, np.array([[2.5, 0.5, 4.9]], np.float32))            # one particle
