@@ -971,7 +971,9 @@ constexpr int kSweepSpan = kSweepCells * kBlock;  // cells per chunk (one workgr
 // 1024 (3.59 us; 2048: 3.79); a paired sweep that computes its diffusion part takes
 // 6.19 us with 1536-2048 workgroups, 6.24 with 1024 and 7.28 with 512, while one that
 // skips it runs the pressure sweep alone, 4.56 us with 1024 against 4.66 with 2048
-// (graph replay of the 128^3 multiphase workload: 794 against 786 steps/s).
+// (graph replay of the 128^3 multiphase workload: 794 against 786 steps/s). With the
+// sweeps' write-through stores, 768 / 1024 / 1280 workgroups: pressure sweep 3.52 / 3.29
+// / 3.33 us, 863 / 901 / 898 steps/s.
 constexpr int kListedGrid = 1024;
 constexpr int kListedGridPaired = 1024;
 
