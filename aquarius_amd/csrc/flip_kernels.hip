@@ -1616,6 +1616,30 @@ __device__ __forceinline__ void paired_chunk(const Geom& G, const PairedFields& 
     }
   }
 }
+// A chunk without FLUID in a full sweep. Interior cells are AIR or FLUID, so all of its
+// cells are AIR: the pressure sweep writes 0 there (jacobi_value of an AIR cell) and a full
+// diffusion sweep copies each cell's own fractions (diffusion_cell's AIR branch). The
+// same stores, without the neighbour, content and right-hand-side loads.
+template <int K>
+__device__ __forceinline__ void air_chunk(const Geom& G, const PairedFields& gr, bool doJ, bool doD, int ch) {
+  SweepCell q[kSweepCells];
+  float v[kSweepCells][K];
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c) {
+    q[c] = sweep_cell(G, ch, c);
+#pragma unroll
+    for (int a = 0; a < K; ++a) v[c][a] = doD ? gr.fin[a][q[c].gi] : 0.f;
+  }
+#pragma unroll
+  for (int c = 0; c < kSweepCells; ++c) {
+    if (!q[c].ok) continue;
+    if (doJ) st_wt(&gr.pout[q[c].gi], 0.f);
+    if (doD) {
+#pragma unroll
+      for (int a = 0; a < K; ++a) st_wt(&gr.fout[a][q[c].gi], v[c][a]);
+    }
+  }
+}
 // Pressure sweep i paired with diffusion sweep i (`sweep` = i). A diffusion sweep that
 // changes no bit leaves both fraction buffers equal (the skipped chunks are AIR,
 // identical in both after the first sweep copied them), and every later sweep then
@@ -1686,7 +1710,9 @@ __global__ void k_jacobi_diffusion(const uint4* __restrict__ tab, const uint32_t
   auto body = [&](int ch) {
     const bool act = !full || chunk_active(f.active, ch);
     const bool doJ = fullJ || act, doD = fullD || act;
-    if (skipD) {  // the pressure sweep alone: no diffusion loads
+    if (!act) {  // a full sweep's chunk without FLUID (skipD implies !fullD: no copy then)
+      air_chunk<K>(G, f, doJ, doD && !skipD, ch);
+    } else if (skipD) {  // the pressure sweep alone: no diffusion loads
       if (doJ) jacobi_chunk<PZERO>(G, f.rhs, f.pin, f.pout, ch);
     } else if (doJ || doD) {
       paired_chunk<K, PZERO>(G, f, 0, 0, lambda, inv_cc, fullD, doJ, doD, ch, changed);
