@@ -802,7 +802,8 @@ __global__ void k_gravity_boundary(float* __restrict__ U, float* __restrict__ V,
   face_of(G, n, x, y, z);
   const long long gi = G.g(x, y, z);
   const uint8_t ct = content[gi];
-  float u = U[gi], v = V[gi], w = W[gi];
+  const float u0 = U[gi], v0 = V[gi], w0 = W[gi];
+  float u = u0, v = v0, w = w0;
   long long ox = x >= 1 ? G.sx : 0, oy = y >= 1 ? (long long)G.EZ : 0, oz = z >= 1 ? 1 : 0;
   asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));  // unconditional loads, one round trip
   const uint8_t cxm = content[gi - ox], cym = content[gi - oy], czm = content[gi - oz];
@@ -846,9 +847,11 @@ __global__ void k_gravity_boundary(float* __restrict__ U, float* __restrict__ V,
     if (cz && z == 0) w = fmaxf(0.f, w);
     if (cz && z == G.Z) { w = fminf(0.f, w); nc = SOLID; }
   }
-  U[gi] = u;
-  V[gi] = v;
-  W[gi] = w;
+  // (only what changed is stored: most faces, away from FLUID and the walls, keep their
+  // values, and the same bits need no store)
+  if (__float_as_uint(u) != __float_as_uint(u0)) U[gi] = u;
+  if (__float_as_uint(v) != __float_as_uint(v0)) V[gi] = v;
+  if (__float_as_uint(w) != __float_as_uint(w0)) W[gi] = w;
   if (nc != ct) content[gi] = nc;
 }
 void launch_gravity_boundary(const Geom& G, Grid gr, int nv, StepParams sp, int doGravity,
