@@ -811,8 +811,8 @@ aq_status pressure_stage(aq_flip* h) {
 
 // Fluid::simulationStep (Fluid/Fluid_3D_FLIP.cu:316-352). G2P, advection, hashing
 // and bin counting are one particle pass; gravity and boundary one face pass.
+// The bin counts are zero on entry: allocated so, and every scan sets them back to zero.
 aq_status enqueue_step(aq_flip* h) {
-  clear_bins(h);
   h->begin(AQ_STAGE_G2P);
   launch_g2p_advect_hash(h->G, h->P[h->cur], h->Np, h->K, h->gr, h->nv, h->nf, h->sp, h->B, h->stream);
   h->end(AQ_STAGE_G2P);

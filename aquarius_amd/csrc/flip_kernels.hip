@@ -246,7 +246,7 @@ __global__ void k_scan_parts(int* __restrict__ partials, int nparts) {
 // round trip; or, when `prefixed`, their sum already formed by k_scan_parts) to its own
 // exclusive scan; the last block also writes offsets[n]. Integer sums, so any
 // association gives the same offsets.
-__global__ void k_scan_final(const int* __restrict__ count, int n, const int* __restrict__ partials,
+__global__ void k_scan_final(int* __restrict__ count, int n, const int* __restrict__ partials,
                              int nparts, int* __restrict__ offsets, int prefixed) {
   __shared__ int wsum[kBlock / 64], wpre[kBlock / 64];
   const int b = blockIdx.x;
@@ -297,6 +297,19 @@ __global__ void k_scan_final(const int* __restrict__ count, int n, const int* __
       if (base + k < n) offsets[base + k] = o[k];
   }
   if (b == nparts - 1 && threadIdx.x == 0) offsets[n] = prefix + tot;
+  // The counts' last reader: the nonzero ones (the cells holding particles, a quarter of
+  // the grid in the 128^3 scenes) are set back to zero, so the next binning needs no
+  // clearing pass. (Zeroing all of them here cost as much as the clearing pass.)
+  if (base + kScanPerThread <= n) {
+    int4* __restrict__ q = reinterpret_cast<int4*>(count + base);
+#pragma unroll
+    for (int k = 0; k < kScanPerThread / 4; ++k)
+      if ((v[4 * k] | v[4 * k + 1] | v[4 * k + 2] | v[4 * k + 3]) != 0) q[k] = make_int4(0, 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k)
+      if (base + k < n && v[k] != 0) count[base + k] = 0;
+  }
 }
 
 void launch_scan(const Geom& G, Bins B, hipStream_t s) {
